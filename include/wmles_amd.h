@@ -1,0 +1,193 @@
+/*
+ * wmles_amd.h -- C ABI of libwmles_amd.so, the MI355X (gfx950) explicit-step hot path
+ * of a compressible wall-modelled LES finite-volume solver.
+ *
+ * The reference exposes this path as Python call signatures only (no FFI); each
+ * entry point below replaces one of them, cited as reference file:line.  Python
+ * binds these symbols with ctypes (wmles_jax_amd/_lib.py).
+ *
+ * Conventions
+ *  - Every pointer argument named q / r / X / vol / xc / S / xf / d_* is DEVICE
+ *    memory allocated and owned by the caller.  The library borrows it for the call
+ *    (geometry pointers: for the plan's lifetime).  The plan owns only its
+ *    metadata (ghost-fill op lists, tile tables), freed by wmx_plan_destroy.
+ *  - State arrays use the reference layout: one float64 array per block of shape
+ *    (5, Ni, Nj, Nk) C-order, Ni = ni + 2 ng, q = [rho, rho u, rho v, rho w, rho E]
+ *    (reference wmles/gas.py:3-4, wmles/mesh.py:3-5).
+ *  - Return value: 0 on success, non-zero on error; wmx_last_error() returns the
+ *    message (thread-local).  No C++ exception crosses the ABI.
+ *  - `stream` is a hipStream_t passed as void* (0 = default stream).  All work is
+ *    enqueued on it; nothing synchronises the host unless stated.
+ *  - A plan is not re-entrant: calls on one plan must not overlap.
+ */
+#ifndef WMLES_AMD_H
+#define WMLES_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WMX_MAX_BLOCKS 64
+
+/* face kinds: reference wmles/mesh.py:77-85 (Interface) and wmles/bc.py:38-166 */
+enum {
+  WMX_FACE_INTERFACE = 0,
+  WMX_BC_SLIP = 1,           /* SlipWall                  bc.py:38-47   */
+  WMX_BC_NOSLIP = 2,         /* NoSlipWall                bc.py:50-64   */
+  WMX_BC_DIRICHLET = 3,      /* Dirichlet                 bc.py:67-77   */
+  WMX_BC_FARFIELD = 4,       /* CharacteristicFarfield    bc.py:80-115  */
+  WMX_BC_EXTRAPOLATE = 5,    /* Extrapolate               bc.py:118-125 */
+  WMX_BC_WALLMODEL = 6,      /* WallModel (slip fill + flux override) bc.py:128-166 */
+  WMX_BC_INFLOW = 7          /* SmoothRampInflow          smooth_ramp.py:116-197 */
+};
+
+/* inviscid schemes: reference wmles/schemes.py */
+enum { WMX_SCHEME_KEP = 0, WMX_SCHEME_KEEP = 1, WMX_SCHEME_MUSCLHLLC = 2, WMX_SCHEME_HYBRID = 3 };
+enum { WMX_LIMITER_ALBADA = 0, WMX_LIMITER_MINMOD = 1 };
+/* SGS models: reference wmles/sgs.py:45-50 */
+enum { WMX_SGS_NONE = 0, WMX_SGS_SMAGORINSKY = 1, WMX_SGS_WALE = 2, WMX_SGS_VREMAN = 3 };
+/* arithmetic forms */
+enum { WMX_PREC_FP64 = 0, WMX_PREC_FP32_BULK = 1 };
+
+typedef struct wmx_face_desc {
+  int32_t kind;
+  int32_t nb;              /* interface: source block index                    */
+  int32_t src_axis;        /* interface: (axis, side) of the face on the source */
+  int32_t src_side;
+  int32_t mapping[3];      /* interface: destination axis t -> source axis     */
+  int32_t flip[3];         /* interface: reverse destination tangential axis t */
+  double shift[3];         /* interface: x_dst = x_src + shift (node exchange)  */
+  double prim[5];          /* Dirichlet / far field: [rho, u, v, w, p]          */
+  double T_wall;           /* NoSlip / WallModel wall temperature; < 0: adiabatic */
+  int32_t n_match;         /* WallModel: matching cell index off the wall (>= 1) */
+  int32_t inflow_id;       /* Inflow: index into the inflow descriptor list     */
+} wmx_face_desc;
+
+typedef struct wmx_block_desc {
+  int32_t n[3];            /* interior cells ni, nj, nk                          */
+  int32_t case_id;         /* ensemble case of this block (0 for a single case)  */
+  wmx_face_desc faces[6];  /* imin, imax, jmin, jmax, kmin, kmax                 */
+  /* geometry (device, reference layout wmles/mesh.py:38-43), borrowed for the
+     plan's lifetime:  vol (Ni,Nj,Nk); xc (3,Ni,Nj,Nk); S[d], xf[d] (3, ..N_d+1..) */
+  const double* vol;
+  const double* xc;
+  const double* S[3];
+  const double* xf[3];
+} wmx_block_desc;
+
+typedef struct wmx_numerics_desc {
+  /* gas: reference wmles/gas.py:10-18 */
+  double gamma, R, Pr, Pr_t, mu_ref, T_ref, S_suth;
+  /* scheme: reference wmles/schemes.py:57-166 */
+  int32_t scheme;          /* WMX_SCHEME_*                                      */
+  int32_t central;         /* Hybrid central part: WMX_SCHEME_KEP or _KEEP      */
+  int32_t order;           /* KEP/KEEP (or Hybrid central) order: 2 or 4        */
+  int32_t limiter;         /* MUSCL limiter: WMX_LIMITER_*                      */
+  double th0, th1;         /* Hybrid ramp                                       */
+  /* SGS: reference wmles/sgs.py */
+  int32_t sgs;             /* WMX_SGS_*                                         */
+  double sgs_c;            /* Cs / Cw / Cs(Vreman)                              */
+  /* wall model: reference wmles/wall_model.py:15-17; the four constants are the
+     host-evaluated exp(-kappa B), Pr^(1/3), Pr^3 and Kader's beta (wall_model.py:21,
+     35, 45, 48) so that they round exactly as the reference's host arithmetic. */
+  double kappa, B;
+  int32_t n_iter;
+  double wm_ekb, wm_pr13, wm_pr3, wm_beta;
+  int32_t bc_passes;       /* reference Solver(bc_passes=2)                     */
+  double cfl;              /* reference Solver(cfl=0.8)                         */
+  int32_t precision;       /* WMX_PREC_*                                        */
+  /* sponge sources (reference smooth_ramp.py:200-221), applied in list order */
+  int32_t n_sponge;
+  double sponge[4][5];     /* x_start_l, x_end_l, strength, U_inf, L            */
+  /* ensemble: per-case constant-mass-flux constraint (reference solver.py:16-36) */
+  int32_t n_cases;
+  int32_t mass_flux_dir[16];   /* -1: no constraint for that case               */
+  double mass_flux_target[16];
+} wmx_numerics_desc;
+
+typedef struct wmx_inflow_desc {
+  /* reference smooth_ramp.py:116-197 (SmoothRampInflow) */
+  int32_t n_tab;           /* rows of the profile table                         */
+  const double* table;     /* device (n_tab, 7): y/L, U, V, uu, vv, ww, uv      */
+  int32_t n_modes;
+  const double* modes;     /* device (3 components, 4, n_modes): kx, ky, kz, phase */
+  double U_inf, L, rho_inf, p_inf;
+} wmx_inflow_desc;
+
+typedef struct wmx_plan wmx_plan;
+
+/* Last error message of the calling thread ("" if none). */
+const char* wmx_last_error(void);
+/* Library build description (arch, flags). */
+const char* wmx_build_info(void);
+
+/* Build a plan: topology, BC parameters, numerics, tile tables and ghost-fill
+   phases.  Replaces the state the reference keeps in Mesh/Solver
+   (wmles/mesh.py:196-242, wmles/solver.py:40-51). */
+int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks,
+                    const wmx_numerics_desc* numerics, int32_t ninflow,
+                    const wmx_inflow_desc* inflows, wmx_plan** out);
+int wmx_plan_destroy(wmx_plan* plan);
+/* Device workspace the plan needs (bytes, 256-aligned); bind it once. */
+int wmx_plan_workspace_bytes(const wmx_plan* plan, size_t* bytes);
+int wmx_plan_bind_workspace(wmx_plan* plan, void* d_workspace, size_t bytes);
+/* Number of WallModel faces and their (block, axis, side, n_t1, n_t2) records. */
+int wmx_plan_wall_faces(const wmx_plan* plan, int32_t* count, int32_t* records /* 5*count or NULL */);
+/* Ghost-fill phases after dependency grouping (diagnostic). */
+int wmx_plan_fill_phases(const wmx_plan* plan, int32_t* nphases);
+
+/* Setup: ghost nodes by node exchange / point reflection, `node_passes` sweeps.
+   X[b]: device (3, Ni+1, Nj+1, Nk+1) with interior nodes set.
+   Replaces wmles/mesh.py:213-232 (Mesh.finalize ghost-node loop, _reflect_nodes). */
+int wmx_fill_nodes(int32_t nblocks, const wmx_block_desc* blocks, double* const* X,
+                   int32_t ng, int32_t node_passes, void* stream);
+/* Setup: face vectors, face centres, volumes and centroids of one block from its
+   ghost-extended nodes.  Replaces wmles/mesh.py:53-73 (compute_geometry). */
+int wmx_compute_geometry(const double* X, const int32_t n_ghosted[3], double* vol, double* xc,
+                         double* S0, double* S1, double* S2,
+                         double* xf0, double* xf1, double* xf2, void* stream);
+
+/* Ghost sweep in place (interior untouched).  Replaces Solver.apply_bcs
+   (wmles/solver.py:59-73). */
+int wmx_apply_bcs(wmx_plan* plan, double* const* q, double time, void* stream);
+/* R = RHS(q) with zero ghosts; q's ghost cells are overwritten by the sweep.
+   Replaces Solver.rhs / _rhs (wmles/solver.py:97-106,159-160). */
+int wmx_rhs(wmx_plan* plan, double* const* q, double* const* r, double time, void* stream);
+/* One SSP-RK3 step + constraints: qout = step(q0).  The three stages run in place
+   in qout (qout must not alias q0).  q0 is returned unchanged (its ghosts are saved
+   to the workspace and restored); qout's ghost cells carry q0's ghost values
+   through the stage combinations with R = 0 there, as in the reference.  dt is
+   read from d_dt[case] (device) when d_dt is non-NULL, else the host value dt.
+   Replaces Solver.advance / _advance_staged (wmles/solver.py:109-133,156-157). */
+int wmx_advance(wmx_plan* plan, double* const* q0, double* const* qout, double dt,
+                const double* d_dt, double time, void* stream);
+/* CFL time step (per case when the plan is an ensemble) into d_dt[n_cases].
+   Replaces Solver.max_dt / _dt (wmles/solver.py:135-154). */
+int wmx_max_dt(wmx_plan* plan, double* const* q, double* d_dt, void* stream);
+/* Constant-mass-flux constraint in place with the plan's per-case targets.
+   Replaces ConstantMassFlux.__call__ (wmles/solver.py:22-36). */
+int wmx_mass_flux(wmx_plan* plan, double* const* q, void* stream);
+/* tau_w and q_w of every WallModel face, concatenated in wmx_plan_wall_faces
+   order, each face (n_t1, n_t2).  Replaces Solver.wall_stats (wmles/solver.py:163-179). */
+int wmx_wall_stats(wmx_plan* plan, double* const* q, double* d_tau, double* d_qw, void* stream);
+
+/* Fill the ng ghost layers of one boundary (non-interface, non-inflow) face of one
+   block in place; synchronises the stream.  The BC protocol's fill(q, blk, geom,
+   gas, axis, side) (reference wmles/bc.py:31-35, _fill_mirror bc.py:14-25). */
+int wmx_fill_face(const wmx_block_desc* block, int32_t axis, int32_t side,
+                  const wmx_numerics_desc* numerics, double* q, double time, void* stream);
+/* Equilibrium wall model on n points (reference wmles/wall_model.py:19-61).
+   mode 0: out0 = u_tau(h=a, U=b, rho_w=c, mu_w=d)              (utau)
+   mode 1: out0 = tau_w, out1 = q_w from (h=a, U=b, T_h=c, p_h=d), T_wall (< 0:
+           adiabatic)                                           (__call__) */
+int wmx_wall_model_eval(const wmx_numerics_desc* numerics, int32_t mode, int64_t n,
+                        const double* a, const double* b, const double* c, const double* d,
+                        double T_wall, double* out0, double* out1, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WMLES_AMD_H */

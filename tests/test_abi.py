@@ -1,0 +1,82 @@
+"""C ABI and host-side logic checks that run without a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:int|const char\*|int32_t)\s+(wmx_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "wmles_jax_amd", "libwmles_amd.so"))
+    names = _declared()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_match_header_sizes():
+    from wmles_jax_amd import _lib
+    import ctypes
+    assert ctypes.sizeof(_lib.FaceDesc) == 120
+    assert ctypes.sizeof(_lib.BlockDesc) == 16 + 6 * 120 + 8 * 8
+
+
+@pytest.mark.parametrize("kind", ["split", "flip", "rot_cyc"])
+def test_orientation_matching_equals_oracle(kind):
+    """Host-side interface orientation (mesh.py:169-193) equals the oracle's."""
+    import wmles_jax_amd.mesh as pm
+    spec, arrays = cases.multiblock_spec(kind, (12, 10, 8))
+    ob = {n: oracle.Block(n, arrays[f"nodes_{n}"]) for n in spec["blocks"]}
+    pb = {n: pm.Block(n, arrays[f"nodes_{n}"]) for n in spec["blocks"]}
+    om, pmesh = oracle.Mesh(list(ob.values())), pm.Mesh(list(pb.values()))
+    for a, fa, b, fb, sh in spec["connect"]:
+        om.connect(ob[a], fa, ob[b], fb, shift=sh)
+        pmesh.connect(pb[a], fa, pb[b], fb, shift=sh)
+    for n in spec["blocks"]:
+        for f, itf in ob[n].faces.items():
+            p = pb[n].faces[f]
+            assert (p.nb, p.src_face, p.mapping, p.flip) == (itf.nb, itf.src_face, itf.mapping, itf.flip)
+            np.testing.assert_array_equal(p.shift, itf.shift)
+
+
+def test_mismatched_interface_raises():
+    import wmles_jax_amd.mesh as pm
+    spec, arrays = cases.multiblock_spec("split", (12, 10, 8))
+    A, B = pm.Block("A", arrays["nodes_A"]), pm.Block("B", arrays["nodes_B"])
+    m = pm.Mesh([A, B])
+    with pytest.raises(ValueError):
+        m.connect(A, "imax", B, "imin", shift=(0.1, 0.0, 0.0))
+
+
+def test_face_descriptor_packing():
+    import wmles_jax_amd as wmx
+    from wmles_jax_amd.mesh import block_desc
+    spec, arrays = cases.multiblock_spec("rot_cyc", (12, 10, 8), ybc={"kind": "WallModel", "T_wall": 300.0,
+                                                                     "n_match": 2})
+    blocks = {n: wmx.Block(n, arrays[f"nodes_{n}"]) for n in spec["blocks"]}
+    m = wmx.Mesh(list(blocks.values()))
+    for a, fa, b, fb, sh in spec["connect"]:
+        m.connect(blocks[a], fa, blocks[b], fb, shift=sh)
+    for nm, face, bc in spec["bcs"]:
+        blocks[nm].set_bc(face, cases.make_bc(wmx, bc))
+    d = block_desc(blocks["B"], m.index)
+    itf = blocks["B"].faces[(2, 0)]
+    fd = d.faces[4]
+    assert fd.kind == 0 and fd.nb == itf.nb and tuple(fd.mapping) == itf.mapping
+    wmf = d.faces[0]
+    assert wmf.kind == 6 and wmf.T_wall == 300.0 and wmf.n_match == 2

@@ -1,0 +1,25 @@
+"""wmles_jax_amd -- MI355X-native explicit time step of a compressible wall-modelled
+LES finite-volume solver, drop-in for the reference's `wmles` step API
+(reference `wmles/__init__.py:1-9`).
+
+The step (ghost sweep, fluxes, gradients, SGS, wall model, SSP-RK3, CFL dt, mass-flux
+constraint) runs in hand-written HIP kernels for gfx950 (libwmles_amd.so, C ABI in
+include/wmles_amd.h).  Importing this package without the built library raises.
+"""
+from . import _lib  # noqa: F401  (fails loudly if libwmles_amd.so is missing)
+from .backend import HAS_JAX
+from .gas import Gas
+from .mesh import Block, Mesh, Geom, Interface, FACE_NAMES, sl
+from .bc import BC, SlipWall, NoSlipWall, Dirichlet, CharacteristicFarfield, Extrapolate, WallModel
+from .wall_model import EquilibriumWallModel
+from .schemes import KEP, KEEP, MUSCLHLLC, Hybrid
+from .sgs import make_sgs
+from .solver import Solver, ConstantMassFlux
+from . import smooth_ramp
+from .smooth_ramp import SmoothRampInflow, SmoothRampSponge, flow_properties
+
+__all__ = ["HAS_JAX", "Gas", "Block", "Mesh", "Geom", "Interface", "FACE_NAMES", "sl", "BC",
+           "SlipWall", "NoSlipWall", "Dirichlet", "CharacteristicFarfield", "Extrapolate",
+           "WallModel", "EquilibriumWallModel", "KEP", "KEEP", "MUSCLHLLC", "Hybrid", "make_sgs",
+           "Solver", "ConstantMassFlux", "smooth_ramp", "SmoothRampInflow", "SmoothRampSponge",
+           "flow_properties"]

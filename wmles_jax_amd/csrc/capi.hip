@@ -1,0 +1,675 @@
+// C ABI of libwmles_amd.so: plan construction (ghost-sweep phases, tile tables,
+// workspace layout) and the launch sequences of the explicit step.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+using namespace wmx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& m) {
+  g_err = m;
+  return 1;
+}
+
+#define WMX_CHECK(expr)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int CI_LEN = 16;          // i-planes per marching tile
+constexpr size_t ALIGN = 256;
+
+size_t align_up(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
+
+struct Box {
+  int blk;
+  int lo[3], hi[3];
+};
+
+bool overlap(const Box& a, const Box& b) {
+  if (a.blk != b.blk) return false;
+  for (int d = 0; d < 3; ++d)
+    if (a.hi[d] <= b.lo[d] || b.hi[d] <= a.lo[d]) return false;
+  return true;
+}
+
+Box full_box(int blk, const int* N) {
+  Box b{blk, {0, 0, 0}, {N[0], N[1], N[2]}};
+  return b;
+}
+
+template <class T>
+int upload(const std::vector<T>& v, T** d) {
+  *d = nullptr;
+  if (v.empty()) return 0;
+  WMX_CHECK(hipMalloc((void**)d, sizeof(T) * v.size()));
+  WMX_CHECK(hipMemcpy(*d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+struct wmx_plan {
+  int nb = 0, ng = 3;
+  std::vector<wmx_block_desc> desc;
+  std::vector<BlockDev> hb;
+  BlockDev* d_blks = nullptr;
+  NumDev nd{};
+  int n_cases = 1;
+  bool any_constraint = false;
+  std::vector<int> case_dir;
+  std::vector<double> case_target;
+  // ghost sweep
+  std::vector<GhostOp> ops;           // all phases, flattened
+  std::vector<int> phase_off;         // size nphases + 1
+  std::vector<int> phase_max;
+  GhostOp* d_ops = nullptr;
+  std::vector<std::vector<int>> phase_inflow;  // block ids of inflow fills per phase
+  // wall faces
+  std::vector<WallOp> wall;
+  WallOp* d_wall = nullptr;
+  int wall_max = 0;
+  // flux tiles
+  std::vector<int4> tiles;
+  int4* d_tiles = nullptr;
+  // per-kernel workgroup prefix tables
+  std::vector<int> pre_all, pre_box, pre_int, pre_shell;
+  int *d_pre_all = nullptr, *d_pre_box = nullptr, *d_pre_int = nullptr, *d_pre_shell = nullptr;
+  // case tables over the interior-cell workgroups
+  std::vector<int> case_wg, case_off;
+  int *d_case_wg = nullptr, *d_case_off = nullptr, *d_case_dir = nullptr;
+  double* d_case_target = nullptr;
+  // inflows
+  std::vector<InflowDev> inflows;
+  // workspace
+  size_t ws_bytes = 0;
+  std::vector<size_t> off_P, off_G, off_shell, off_sponge;
+  std::vector<std::vector<long long>> off_wall;   // per block, 6 faces (-1: none)
+  size_t off_part = 0, off_scal = 0;
+  char* ws = nullptr;
+  double* d_part = nullptr;
+  double* d_delta = nullptr;
+};
+
+namespace {
+
+int nwg_of(const std::vector<int>& pre) { return pre.empty() ? 0 : pre.back(); }
+
+std::vector<int> prefix(const std::vector<long long>& counts) {
+  std::vector<int> p(counts.size() + 1, 0);
+  for (size_t i = 0; i < counts.size(); ++i) p[i + 1] = p[i] + (int)((counts[i] + 255) / 256);
+  return p;
+}
+
+// read / write boxes of one fill op (sequential-sweep semantics of solver.py:59-73)
+void op_boxes(const wmx_plan& p, int b, int a, int s, std::vector<Box>* rd, Box* wr) {
+  const BlockDev& B = p.hb[b];
+  const int ng = B.ng, n = B.n[a];
+  *wr = full_box(b, B.N);
+  wr->lo[a] = s == 0 ? 0 : ng + n;
+  wr->hi[a] = s == 0 ? ng : B.N[a];
+  const wmx_face_desc& f = B.faces[2 * a + s];
+  rd->clear();
+  if (f.kind == WMX_FACE_INTERFACE) {
+    const BlockDev& S = p.hb[f.nb];
+    Box r = full_box(f.nb, S.N);
+    const int aB = f.src_axis, nB = S.n[aB];
+    r.lo[aB] = f.src_side == 0 ? ng : nB;
+    r.hi[aB] = r.lo[aB] + ng;
+    rd->push_back(r);
+  } else if (f.kind == WMX_BC_DIRICHLET || f.kind == WMX_BC_INFLOW) {
+    // reads no state
+  } else {
+    Box r = full_box(b, B.N);
+    if (f.kind == WMX_BC_EXTRAPOLATE) {
+      r.lo[a] = s == 0 ? ng : ng + n - 1;
+      r.hi[a] = r.lo[a] + 1;
+    } else {
+      r.lo[a] = s == 0 ? ng : n;
+      r.hi[a] = r.lo[a] + ng;
+    }
+    rd->push_back(r);
+  }
+}
+
+int build_phases(wmx_plan& p, int passes) {
+  struct Pending {
+    GhostOp op;
+    std::vector<Box> rd;
+    Box wr;
+  };
+  std::vector<Pending> cur;
+  p.ops.clear();
+  p.phase_off.assign(1, 0);
+  p.phase_max.clear();
+  p.phase_inflow.clear();
+  auto close = [&]() {
+    if (cur.empty()) return;
+    int mx = 0;
+    std::vector<int> inflow;
+    for (auto& c : cur) {
+      p.ops.push_back(c.op);
+      mx = std::max(mx, c.op.count);
+      if (p.hb[c.op.blk].faces[2 * c.op.axis + c.op.side].kind == WMX_BC_INFLOW)
+        inflow.push_back(c.op.blk);
+    }
+    p.phase_off.push_back((int)p.ops.size());
+    p.phase_max.push_back(mx);
+    p.phase_inflow.push_back(inflow);
+    cur.clear();
+  };
+  for (int pass = 0; pass < passes; ++pass)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < p.nb; ++b)
+        for (int s = 0; s < 2; ++s) {
+          Pending x;
+          const BlockDev& B = p.hb[b];
+          int t1, t2;
+          tangents(a, &t1, &t2);
+          x.op = GhostOp{b, a, s, B.ng * B.N[t1] * B.N[t2]};
+          op_boxes(p, b, a, s, &x.rd, &x.wr);
+          bool conflict = false;
+          for (auto& c : cur) {
+            if (overlap(c.wr, x.wr)) conflict = true;
+            for (auto& r : x.rd) if (overlap(c.wr, r)) conflict = true;
+            for (auto& r : c.rd) if (overlap(x.wr, r)) conflict = true;
+            if (conflict) break;
+          }
+          if (conflict) close();
+          cur.push_back(x);
+        }
+  close();
+  return 0;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+NumDev make_numdev(const wmx_numerics_desc* nm) {
+  NumDev nd;
+  std::memset(&nd, 0, sizeof(nd));
+  nd.gas = GasC{nm->gamma, nm->R, nm->Pr, nm->Pr_t, nm->mu_ref, nm->T_ref, nm->S_suth,
+                nm->gamma * nm->R / (nm->gamma - 1.0)};
+  nd.wm = WallModelC{nm->kappa, nm->B, nm->wm_ekb, nm->wm_pr13, nm->wm_pr3, nm->wm_beta, nm->n_iter};
+  nd.scheme = nm->scheme;
+  nd.central = nm->central;
+  nd.order = nm->order;
+  nd.limiter = nm->limiter;
+  nd.th0 = nm->th0;
+  nd.th1 = nm->th1;
+  nd.sgs = nm->sgs;
+  nd.sgs_c = nm->sgs_c;
+  nd.cfl = nm->cfl;
+  nd.n_sponge = nm->n_sponge;
+  std::memcpy(nd.sponge, nm->sponge, sizeof(nd.sponge));
+  return nd;
+}
+
+Ptrs make_ptrs(const wmx_plan* p, double* const* q) {
+  Ptrs r;
+  std::memset(&r, 0, sizeof(r));
+  for (int b = 0; b < p->nb; ++b) r.p[b] = q[b];
+  return r;
+}
+
+int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s) {
+  const int nph = (int)p->phase_max.size();
+  for (int ph = 0; ph < nph; ++ph) {
+    const int o = p->phase_off[ph], n = p->phase_off[ph + 1] - o;
+    WMX_CHECK(launch_fill_cells(p->d_blks, p->d_ops + o, n, p->phase_max[ph], q, p->nd.gas, s));
+    for (int blk : p->phase_inflow[ph]) {
+      const BlockDev& B = p->hb[blk];
+      const int id = B.faces[0].inflow_id;
+      WMX_CHECK(launch_inflow(p->d_blks, blk, B.N[1], B.N[2], B.ng, p->inflows[id], q, p->nd.gas,
+                              time, d_dt, coef, s));
+    }
+  }
+  return 0;
+}
+
+// ghost fill + cell-level fields of one RHS evaluation
+int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s) {
+  if (run_fill(p, q, time, d_dt, coef, s)) return 1;
+  WMX_CHECK(launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd.gas, s));
+  WMX_CHECK(launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s));
+  WMX_CHECK(launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s));
+  if (p->nd.n_sponge > 0)
+    for (int b = 0; b < p->nb; ++b)
+      WMX_CHECK(launch_sponge_mean(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], q, s));
+  return 0;
+}
+
+int check_plan(const wmx_plan* p) {
+  if (!p) return fail("null plan");
+  if (!p->ws) return fail("plan has no workspace bound (wmx_plan_bind_workspace)");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* wmx_last_error(void) { return g_err.c_str(); }
+
+const char* wmx_build_info(void) {
+  return "libwmles_amd: gfx950, FP64 plain form, hipcc -O3, no packed-fp32";
+}
+
+int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_numerics_desc* nm,
+                    int32_t ninflow, const wmx_inflow_desc* inflows, wmx_plan** out) {
+  if (!out || !blocks || !nm) return fail("null argument");
+  *out = nullptr;
+  if (nblocks < 1 || nblocks > WMX_MAX_BLOCKS)
+    return fail("nblocks must be in [1, " + std::to_string(WMX_MAX_BLOCKS) + "]");
+  if (nm->n_cases < 1 || nm->n_cases > 16) return fail("n_cases must be in [1, 16]");
+  if (nm->n_sponge < 0 || nm->n_sponge > 4) return fail("at most 4 sponge sources");
+  if (nm->order != 2 && nm->order != 4) return fail("central order must be 2 or 4");
+  if (nm->bc_passes < 1) return fail("bc_passes must be >= 1");
+  wmx_plan* p = new wmx_plan();
+  p->nb = nblocks;
+  p->desc.assign(blocks, blocks + nblocks);
+  NumDev& nd = p->nd;
+  nd = make_numdev(nm);
+  p->n_cases = nm->n_cases;
+  p->case_dir.assign(nm->mass_flux_dir, nm->mass_flux_dir + 16);
+  p->case_target.assign(nm->mass_flux_target, nm->mass_flux_target + 16);
+  for (int c = 0; c < p->n_cases; ++c) p->any_constraint |= p->case_dir[c] >= 0;
+  // blocks
+  p->hb.resize(nblocks);
+  const int ng = 3;
+  p->ng = ng;
+  for (int b = 0; b < nblocks; ++b) {
+    const wmx_block_desc& d = blocks[b];
+    BlockDev& B = p->hb[b];
+    std::memset(&B, 0, sizeof(B));
+    for (int a = 0; a < 3; ++a) {
+      if (d.n[a] < ng) { delete p; return fail("each block needs at least ng cells per direction"); }
+      B.n[a] = d.n[a];
+      B.N[a] = d.n[a] + 2 * ng;
+    }
+    B.ng = ng;
+    B.case_id = d.case_id;
+    if (d.case_id < 0 || d.case_id >= p->n_cases) { delete p; return fail("case_id out of range"); }
+    B.cells = (long long)B.N[0] * B.N[1] * B.N[2];
+    B.vol = d.vol;
+    B.xc = d.xc;
+    for (int a = 0; a < 3; ++a) { B.S[a] = d.S[a]; B.xf[a] = d.xf[a]; }
+    if (!B.vol || !B.xc || !B.S[0] || !B.S[1] || !B.S[2] || !B.xf[0] || !B.xf[1] || !B.xf[2]) {
+      delete p;
+      return fail("block geometry pointers must be set");
+    }
+    B.shell_cells = shell_count(B.N, B.n, ng);
+    for (int f = 0; f < 6; ++f) B.faces[f] = d.faces[f];
+  }
+  // validate faces
+  for (int b = 0; b < nblocks; ++b)
+    for (int f = 0; f < 6; ++f) {
+      const wmx_face_desc& fc = p->hb[b].faces[f];
+      const int a = f / 2;
+      if (fc.kind == WMX_FACE_INTERFACE) {
+        if (fc.nb < 0 || fc.nb >= nblocks) { delete p; return fail("interface source block out of range"); }
+        const BlockDev& S = p->hb[fc.nb];
+        for (int t = 0; t < 3; ++t) {
+          if (t == a) continue;
+          if (S.N[fc.mapping[t]] != p->hb[b].N[t]) { delete p; return fail("interface extents do not match"); }
+        }
+        if (fc.mapping[a] != fc.src_axis) { delete p; return fail("interface mapping inconsistent"); }
+      } else if (fc.kind == WMX_BC_INFLOW) {
+        if (f != 0) { delete p; return fail("inflow is only valid on the i-min face"); }
+        if (fc.inflow_id < 0 || fc.inflow_id >= ninflow) { delete p; return fail("inflow_id out of range"); }
+        if (p->hb[b].N[2] > 1024) { delete p; return fail("inflow fill supports N_k <= 1024"); }
+      } else if (fc.kind == WMX_BC_WALLMODEL) {
+        if (fc.n_match < 1 || fc.n_match > p->hb[b].n[a]) { delete p; return fail("n_match out of range"); }
+      } else if (fc.kind < 0 || fc.kind > WMX_BC_INFLOW) {
+        delete p;
+        return fail("unknown face kind");
+      }
+    }
+  for (int i = 0; i < ninflow; ++i) {
+    const wmx_inflow_desc& in = inflows[i];
+    p->inflows.push_back(InflowDev{in.n_tab, in.n_modes, in.table, in.modes, in.U_inf, in.L,
+                                   in.rho_inf, in.p_inf});
+  }
+  build_phases(*p, nm->bc_passes);
+  // wall faces
+  for (int b = 0; b < nblocks; ++b)
+    for (int f = 0; f < 6; ++f)
+      if (p->hb[b].faces[f].kind == WMX_BC_WALLMODEL) {
+        int t1, t2;
+        tangents(f / 2, &t1, &t2);
+        int cnt = p->hb[b].n[t1] * p->hb[b].n[t2];
+        p->wall.push_back(WallOp{b, f / 2, f % 2, cnt});
+        p->wall_max = std::max(p->wall_max, cnt);
+      }
+  // flux tiles: per block (i-chunk, j-tile, k-tile)
+  for (int b = 0; b < nblocks; ++b) {
+    const BlockDev& B = p->hb[b];
+    for (int i0 = 0; i0 < B.n[0]; i0 += CI_LEN)
+      for (int j0 = 0; j0 < B.n[1]; j0 += 16)
+        for (int k0 = 0; k0 < B.n[2]; k0 += 16) p->tiles.push_back(make_int4(b, j0, k0, i0));
+  }
+  // prefix tables
+  std::vector<long long> c_all, c_box, c_int, c_shell;
+  for (auto& B : p->hb) {
+    c_all.push_back(B.cells);
+    c_box.push_back((long long)(B.n[0] + 2) * (B.n[1] + 2) * (B.n[2] + 2));
+    c_int.push_back((long long)B.n[0] * B.n[1] * B.n[2]);
+    c_shell.push_back(B.shell_cells);
+  }
+  p->pre_all = prefix(c_all);
+  p->pre_box = prefix(c_box);
+  p->pre_int = prefix(c_int);
+  p->pre_shell = prefix(c_shell);
+  p->case_off.assign(1, 0);
+  for (int c = 0; c < p->n_cases; ++c) {
+    for (int b = 0; b < nblocks; ++b)
+      if (p->hb[b].case_id == c)
+        for (int w = p->pre_int[b]; w < p->pre_int[b + 1]; ++w) p->case_wg.push_back(w);
+    p->case_off.push_back((int)p->case_wg.size());
+  }
+  // workspace layout
+  size_t off = 0;
+  p->off_wall.assign(nblocks, std::vector<long long>(6, -1));
+  for (int b = 0; b < nblocks; ++b) {
+    const BlockDev& B = p->hb[b];
+    p->off_P.push_back(off);
+    off = align_up(off + sizeof(double) * NP * B.cells);
+    p->off_G.push_back(off);
+    off = align_up(off + sizeof(double) * NGR * B.cells);
+    p->off_shell.push_back(off);
+    off = align_up(off + sizeof(double) * 5 * B.shell_cells);
+    p->off_sponge.push_back(off);
+    if (nd.n_sponge > 0) off = align_up(off + sizeof(double) * 5 * B.n[0] * B.n[1]);
+    for (int f = 0; f < 6; ++f)
+      if (B.faces[f].kind == WMX_BC_WALLMODEL) {
+        int t1, t2;
+        tangents(f / 2, &t1, &t2);
+        p->off_wall[b][f] = (long long)off;
+        off = align_up(off + sizeof(double) * WALL_PLANES * B.n[t1] * B.n[t2]);
+      }
+  }
+  p->off_part = off;
+  off = align_up(off + sizeof(double) * 2 * (size_t)std::max(1, nwg_of(p->pre_int)));
+  p->off_scal = off;
+  off = align_up(off + sizeof(double) * 64);
+  p->ws_bytes = off;
+  // device metadata
+  std::vector<int> cdir(p->case_dir.begin(), p->case_dir.end());
+  if (upload(p->ops, &p->d_ops) || upload(p->wall, &p->d_wall) || upload(p->tiles, &p->d_tiles) ||
+      upload(p->pre_all, &p->d_pre_all) || upload(p->pre_box, &p->d_pre_box) ||
+      upload(p->pre_int, &p->d_pre_int) || upload(p->pre_shell, &p->d_pre_shell) ||
+      upload(p->case_wg, &p->d_case_wg) || upload(p->case_off, &p->d_case_off) ||
+      upload(cdir, &p->d_case_dir) || upload(p->case_target, &p->d_case_target)) {
+    std::string m = g_err;
+    wmx_plan_destroy(p);
+    return fail(m);
+  }
+  if (hipMalloc((void**)&p->d_blks, sizeof(BlockDev) * nblocks) != hipSuccess) {
+    wmx_plan_destroy(p);
+    return fail("hipMalloc of block descriptors failed");
+  }
+  *out = p;
+  return 0;
+}
+
+int wmx_plan_destroy(wmx_plan* p) {
+  if (!p) return 0;
+  (void)hipFree(p->d_blks);
+  (void)hipFree(p->d_ops);
+  (void)hipFree(p->d_wall);
+  (void)hipFree(p->d_tiles);
+  (void)hipFree(p->d_pre_all);
+  (void)hipFree(p->d_pre_box);
+  (void)hipFree(p->d_pre_int);
+  (void)hipFree(p->d_pre_shell);
+  (void)hipFree(p->d_case_wg);
+  (void)hipFree(p->d_case_off);
+  (void)hipFree(p->d_case_dir);
+  (void)hipFree(p->d_case_target);
+  delete p;
+  return 0;
+}
+
+int wmx_plan_workspace_bytes(const wmx_plan* p, size_t* bytes) {
+  if (!p || !bytes) return fail("null argument");
+  *bytes = p->ws_bytes;
+  return 0;
+}
+
+int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
+  if (!p || !ws) return fail("null argument");
+  if (bytes < p->ws_bytes) return fail("workspace too small");
+  if (reinterpret_cast<uintptr_t>(ws) % ALIGN) return fail("workspace must be 256-byte aligned");
+  p->ws = static_cast<char*>(ws);
+  for (int b = 0; b < p->nb; ++b) {
+    BlockDev& B = p->hb[b];
+    B.P = reinterpret_cast<double*>(p->ws + p->off_P[b]);
+    B.G = reinterpret_cast<double*>(p->ws + p->off_G[b]);
+    B.shell = reinterpret_cast<double*>(p->ws + p->off_shell[b]);
+    B.sponge_mean = reinterpret_cast<double*>(p->ws + p->off_sponge[b]);
+    for (int f = 0; f < 6; ++f)
+      B.wall[f] = p->off_wall[b][f] >= 0 ? reinterpret_cast<double*>(p->ws + p->off_wall[b][f]) : nullptr;
+  }
+  p->d_part = reinterpret_cast<double*>(p->ws + p->off_part);
+  p->d_delta = reinterpret_cast<double*>(p->ws + p->off_scal);
+  WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int wmx_plan_wall_faces(const wmx_plan* p, int32_t* count, int32_t* rec) {
+  if (!p || !count) return fail("null argument");
+  *count = (int32_t)p->wall.size();
+  if (rec)
+    for (size_t i = 0; i < p->wall.size(); ++i) {
+      const WallOp& w = p->wall[i];
+      int t1, t2;
+      tangents(w.axis, &t1, &t2);
+      rec[5 * i + 0] = w.blk;
+      rec[5 * i + 1] = w.axis;
+      rec[5 * i + 2] = w.side;
+      rec[5 * i + 3] = p->hb[w.blk].n[t1];
+      rec[5 * i + 4] = p->hb[w.blk].n[t2];
+    }
+  return 0;
+}
+
+int wmx_plan_fill_phases(const wmx_plan* p, int32_t* nphases) {
+  if (!p || !nphases) return fail("null argument");
+  *nphases = (int32_t)p->phase_max.size();
+  return 0;
+}
+
+int wmx_fill_nodes(int32_t nblocks, const wmx_block_desc* blocks, double* const* X, int32_t ng,
+                   int32_t node_passes, void* stream) {
+  if (nblocks < 1 || nblocks > WMX_MAX_BLOCKS || !blocks || !X) return fail("bad arguments");
+  if (ng != 3) return fail("ng must be 3");
+  hipStream_t s = as_stream(stream);
+  std::vector<BlockDev> hb(nblocks);
+  for (int b = 0; b < nblocks; ++b) {
+    BlockDev& B = hb[b];
+    std::memset(&B, 0, sizeof(B));
+    for (int a = 0; a < 3; ++a) { B.n[a] = blocks[b].n[a]; B.N[a] = B.n[a] + 2 * ng; }
+    B.ng = ng;
+    B.cells = (long long)B.N[0] * B.N[1] * B.N[2];
+    for (int f = 0; f < 6; ++f) B.faces[f] = blocks[b].faces[f];
+  }
+  BlockDev* d_blks = nullptr;
+  GhostOp* d_ops = nullptr;
+  std::vector<GhostOp> ops;
+  for (int pass = 0; pass < node_passes; ++pass)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < nblocks; ++b)
+        for (int sd = 0; sd < 2; ++sd) {
+          int t1, t2;
+          tangents(a, &t1, &t2);
+          ops.push_back(GhostOp{b, a, sd, ng * (hb[b].N[t1] + 1) * (hb[b].N[t2] + 1)});
+        }
+  WMX_CHECK(hipMalloc((void**)&d_blks, sizeof(BlockDev) * nblocks));
+  WMX_CHECK(hipMemcpy(d_blks, hb.data(), sizeof(BlockDev) * nblocks, hipMemcpyHostToDevice));
+  if (upload(ops, &d_ops)) { (void)hipFree(d_blks); return 1; }
+  Ptrs xp;
+  std::memset(&xp, 0, sizeof(xp));
+  for (int b = 0; b < nblocks; ++b) xp.p[b] = X[b];
+  int rc = 0;
+  for (size_t i = 0; i < ops.size() && !rc; ++i) {
+    hipError_t e = launch_fill_nodes(d_blks, d_ops + i, 1, ops[i].count, xp, s);
+    if (e != hipSuccess) rc = fail(std::string("node fill: ") + hipGetErrorString(e));
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (!rc && e != hipSuccess) rc = fail(std::string("node fill: ") + hipGetErrorString(e));
+  (void)hipFree(d_blks);
+  (void)hipFree(d_ops);
+  return rc;
+}
+
+int wmx_compute_geometry(const double* X, const int32_t N[3], double* vol, double* xc, double* S0,
+                         double* S1, double* S2, double* xf0, double* xf1, double* xf2,
+                         void* stream) {
+  if (!X || !N || !vol || !xc || !S0 || !S1 || !S2 || !xf0 || !xf1 || !xf2) return fail("null argument");
+  int n3[3] = {N[0], N[1], N[2]};
+  double* S[3] = {S0, S1, S2};
+  double* xf[3] = {xf0, xf1, xf2};
+  WMX_CHECK(launch_geometry(X, n3, vol, xc, S, xf, as_stream(stream)));
+  return 0;
+}
+
+int wmx_apply_bcs(wmx_plan* p, double* const* q, double time, void* stream) {
+  if (check_plan(p)) return 1;
+  return run_fill(p, make_ptrs(p, q), time, nullptr, 0.0, as_stream(stream));
+}
+
+int wmx_rhs(wmx_plan* p, double* const* q, double* const* r, double time, void* stream) {
+  if (check_plan(p)) return 1;
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q), pr = make_ptrs(p, r);
+  if (run_cells(p, pq, time, nullptr, 0.0, s)) return 1;
+  WMX_CHECK(launch_flux(0, p->d_blks, p->d_tiles, (int)p->tiles.size(), CI_LEN, p->nd, pq, pq, pr,
+                        nullptr, 0.0, s));
+  WMX_CHECK(launch_shell_finish(0, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), pq, pr,
+                                nullptr, 0.0, s));
+  return 0;
+}
+
+int wmx_mass_flux(wmx_plan* p, double* const* q, void* stream) {
+  if (check_plan(p)) return 1;
+  if (!p->any_constraint) return 0;
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q);
+  WMX_CHECK(launch_mf_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq, p->d_case_dir,
+                              p->d_part, s));
+  WMX_CHECK(launch_case_final(1, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part,
+                              p->d_case_target, p->d_delta, s));
+  WMX_CHECK(launch_mf_apply(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), pq, p->d_case_dir,
+                            p->d_delta, s));
+  return 0;
+}
+
+int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, const double* d_dt,
+                double time, void* stream) {
+  if (check_plan(p)) return 1;
+  for (int b = 0; b < p->nb; ++b)
+    if (q0[b] == qout[b]) return fail("qout must not alias q0");
+  hipStream_t s = as_stream(stream);
+  Ptrs P0 = make_ptrs(p, q0), PO = make_ptrs(p, qout);
+  const int ntiles = (int)p->tiles.size();
+  WMX_CHECK(launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s));
+  // stage times t, t + dt, t + dt/2 (solver.py:125-129)
+  const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
+  if (run_cells(p, P0, time, d_dt, 0.0, s)) return 1;
+  WMX_CHECK(launch_flux(1, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, P0, P0, PO, d_dt, dt, s));
+  if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
+  WMX_CHECK(launch_flux(2, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s));
+  if (run_cells(p, PO, t2, d_dt, 0.5, s)) return 1;
+  WMX_CHECK(launch_flux(3, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s));
+  WMX_CHECK(launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO,
+                                d_dt, dt, s));
+  return wmx_mass_flux(p, qout, stream);
+}
+
+int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
+  if (check_plan(p)) return 1;
+  if (!d_dt) return fail("null d_dt");
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q);
+  WMX_CHECK(launch_dt_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq, p->nd,
+                              p->d_part, s));
+  WMX_CHECK(launch_case_final(0, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part, nullptr, d_dt, s));
+  return 0;
+}
+
+int wmx_wall_stats(wmx_plan* p, double* const* q, double* d_tau, double* d_qw, void* stream) {
+  if (check_plan(p)) return 1;
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q);
+  WMX_CHECK(launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, pq, p->nd, s));
+  size_t off = 0;
+  for (const WallOp& w : p->wall) {
+    const double* W = p->hb[w.blk].wall[2 * w.axis + w.side];
+    WMX_CHECK(hipMemcpyAsync(d_tau + off, W + 5 * (size_t)w.count, sizeof(double) * w.count,
+                             hipMemcpyDeviceToDevice, s));
+    WMX_CHECK(hipMemcpyAsync(d_qw + off, W + 6 * (size_t)w.count, sizeof(double) * w.count,
+                             hipMemcpyDeviceToDevice, s));
+    off += w.count;
+  }
+  return 0;
+}
+
+int wmx_fill_face(const wmx_block_desc* blk, int32_t axis, int32_t side, const wmx_numerics_desc* nm,
+                  double* q, double time, void* stream) {
+  (void)time;
+  if (!blk || !nm || !q) return fail("null argument");
+  if (axis < 0 || axis > 2 || side < 0 || side > 1) return fail("bad face");
+  const wmx_face_desc& f = blk->faces[2 * axis + side];
+  if (f.kind == WMX_FACE_INTERFACE || f.kind == WMX_BC_INFLOW || f.kind < 0 || f.kind > WMX_BC_INFLOW)
+    return fail("wmx_fill_face handles boundary-condition faces only");
+  const int ng = 3;
+  BlockDev B;
+  std::memset(&B, 0, sizeof(B));
+  for (int a = 0; a < 3; ++a) { B.n[a] = blk->n[a]; B.N[a] = blk->n[a] + 2 * ng; }
+  B.ng = ng;
+  B.cells = (long long)B.N[0] * B.N[1] * B.N[2];
+  B.vol = blk->vol;
+  B.xc = blk->xc;
+  for (int a = 0; a < 3; ++a) { B.S[a] = blk->S[a]; B.xf[a] = blk->xf[a]; }
+  for (int k = 0; k < 6; ++k) B.faces[k] = blk->faces[k];
+  if (!B.S[axis]) return fail("face vectors required");
+  int t1, t2;
+  tangents(axis, &t1, &t2);
+  GhostOp op{0, axis, side, ng * B.N[t1] * B.N[t2]};
+  hipStream_t s = as_stream(stream);
+  BlockDev* d_b = nullptr;
+  GhostOp* d_op = nullptr;
+  WMX_CHECK(hipMalloc((void**)&d_b, sizeof(BlockDev)));
+  WMX_CHECK(hipMalloc((void**)&d_op, sizeof(GhostOp)));
+  WMX_CHECK(hipMemcpy(d_b, &B, sizeof(BlockDev), hipMemcpyHostToDevice));
+  WMX_CHECK(hipMemcpy(d_op, &op, sizeof(GhostOp), hipMemcpyHostToDevice));
+  Ptrs pq;
+  std::memset(&pq, 0, sizeof(pq));
+  pq.p[0] = q;
+  NumDev nd = make_numdev(nm);
+  hipError_t e = launch_fill_cells(d_b, d_op, 1, op.count, pq, nd.gas, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d_b);
+  (void)hipFree(d_op);
+  if (e != hipSuccess) return fail(std::string("fill_face: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int wmx_wall_model_eval(const wmx_numerics_desc* nm, int32_t mode, int64_t n, const double* a,
+                        const double* b, const double* c, const double* d, double T_wall,
+                        double* o0, double* o1, void* stream) {
+  if (!nm || !a || !b || !c || !d || !o0 || (mode == 1 && !o1)) return fail("null argument");
+  if (mode != 0 && mode != 1) return fail("mode must be 0 or 1");
+  NumDev nd = make_numdev(nm);
+  WMX_CHECK(launch_wm_eval(nd, mode, n, a, b, c, d, T_wall, o0, o1, as_stream(stream)));
+  return 0;
+}
+
+}  // extern "C"

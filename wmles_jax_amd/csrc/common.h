@@ -1,0 +1,219 @@
+// Internal device-side descriptors shared by the kernels and the C-ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wmles_amd.h"
+#include "physics.h"
+
+namespace wmx {
+
+constexpr int NP = 8;    // prim planes: rho u v w p T mu e_int
+constexpr int NGR = 15;  // gradient record planes: g[4][3], mu_eff, k_eff, theta
+constexpr int WALL_PLANES = 7;  // Fw[5], tau, qw
+
+enum PrimPlane { P_R = 0, P_U, P_V, P_W, P_P, P_T, P_MU, P_E };
+enum GradPlane { G_MU = 12, G_K = 13, G_TH = 14 };
+
+struct Ptrs {
+  double* p[WMX_MAX_BLOCKS];
+};
+
+struct BlockDev {
+  int n[3];
+  int N[3];
+  int ng;
+  int case_id;
+  long long cells;          // N0 N1 N2
+  const double* vol;
+  const double* xc;
+  const double* S[3];
+  const double* xf[3];
+  double* P;                // NP planes
+  double* G;                // NGR planes
+  double* wall[6];          // WALL_PLANES planes of (n_t1, n_t2) per WallModel face
+  double* sponge_mean;      // (5, n0, n1)
+  double* shell;            // (5, shell cells) saved ghost values
+  long long shell_cells;
+  wmx_face_desc faces[6];
+};
+
+struct NumDev {
+  GasC gas;
+  WallModelC wm;
+  int scheme, central, order, limiter;
+  double th0, th1;
+  int sgs;
+  double sgs_c;
+  double cfl;
+  int n_sponge;
+  double sponge[4][5];
+};
+
+struct GhostOp {
+  int blk;       // destination block
+  int axis, side;
+  int count;     // ng * N_t1 * N_t2 ghost entries (cells or nodes)
+};
+
+struct WallOp {
+  int blk, axis, side, count;
+};
+
+struct InflowDev {
+  int n_tab, n_modes;
+  const double* table;
+  const double* modes;
+  double U_inf, L, rho_inf, p_inf;
+};
+
+// host-side launchers (one per kernel family; defined beside the kernels)
+hipError_t launch_fill_nodes(const BlockDev* d_blks, const GhostOp* d_ops, int nops, int maxcount,
+                             Ptrs X, hipStream_t s);
+hipError_t launch_fill_cells(const BlockDev* d_blks, const GhostOp* d_ops, int nops, int maxcount,
+                             Ptrs q, GasC gas, hipStream_t s);
+hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng, InflowDev in,
+                         Ptrs q, GasC gas, double time, const double* d_dt, double coef,
+                         hipStream_t s);
+hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
+                           double* const* xf, hipStream_t s);
+hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q, GasC gas,
+                       hipStream_t s);
+hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
+                       hipStream_t s);
+hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, int maxcount, Ptrs q,
+                       const NumDev& nd, hipStream_t s);
+hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles, int ci_len,
+                       const NumDev& nd, Ptrs qin, Ptrs q0, Ptrs out, const double* d_dt,
+                       double dt, hipStream_t s);
+hipError_t launch_shell_save(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                             hipStream_t s);
+hipError_t launch_shell_finish(int mode, const BlockDev* d_blks, const int* d_pre, int nblk, int nwg,
+                               Ptrs q0, Ptrs out, const double* d_dt, double dt, hipStream_t s);
+hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                             const NumDev& nd, double* d_part, hipStream_t s);
+hipError_t launch_mf_partial(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                             const int* d_case_dir, double* d_part, hipStream_t s);
+hipError_t launch_case_final(int mode, int ncases, const int* d_case_wg, const int* d_case_off,
+                             const double* d_part, const double* d_target, double* d_out,
+                             hipStream_t s);
+hipError_t launch_mf_apply(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                           const int* d_case_dir, const double* d_delta, hipStream_t s);
+hipError_t launch_wm_eval(const NumDev& nd, int mode, long long n, const double* a, const double* b,
+                          const double* c, const double* d, double T_wall, double* o0, double* o1,
+                          hipStream_t s);
+hipError_t launch_sponge_mean(const BlockDev* d_blks, int blk, long long rows, Ptrs q,
+                              hipStream_t s);
+
+// tangential axes of a face normal to `axis`, increasing order
+WMX_HD void tangents(int axis, int* t1, int* t2) {
+  *t1 = axis == 0 ? 1 : 0;
+  *t2 = axis == 2 ? 1 : 2;
+}
+
+WMX_HD long long cidx(const int* N, int I, int J, int K) {
+  return ((long long)I * N[1] + J) * N[2] + K;
+}
+
+// face-vector array S[d] has N_d + 1 entries along d
+WMX_HD long long fidx(const int* N, int d, int I, int J, int K) {
+  int M1 = N[1] + (d == 1), M2 = N[2] + (d == 2);
+  return ((long long)I * M1 + J) * M2 + K;
+}
+WMX_HD long long fcount(const int* N, int d) {
+  return (long long)(N[0] + (d == 0)) * (N[1] + (d == 1)) * (N[2] + (d == 2));
+}
+
+// ghost shell cell s -> (I, J, K): i-slabs (full j,k), j-slabs (interior i), k-slabs
+WMX_HD void shell_cell(long long s, const int* N, const int* n, int ng, int* I, int* J, int* K) {
+  long long a = 2LL * ng * N[1] * N[2];
+  if (s < a) {
+    long long l = s / ((long long)N[1] * N[2]);
+    long long r = s % ((long long)N[1] * N[2]);
+    *I = l < ng ? (int)l : (int)(n[0] + l);
+    *J = (int)(r / N[2]);
+    *K = (int)(r % N[2]);
+    return;
+  }
+  s -= a;
+  long long b = 2LL * ng * n[0] * N[2];
+  if (s < b) {
+    long long per = (long long)n[0] * N[2];
+    long long l = s / per;
+    long long r = s % per;
+    *I = ng + (int)(r / N[2]);
+    *K = (int)(r % N[2]);
+    *J = l < ng ? (int)l : (int)(n[1] + l);
+    return;
+  }
+  s -= b;
+  long long per = (long long)n[0] * n[1];
+  long long l = s / per;
+  long long r = s % per;
+  *I = ng + (int)(r / n[1]);
+  *J = ng + (int)(r % n[1]);
+  *K = l < ng ? (int)l : (int)(n[2] + l);
+}
+
+WMX_HD long long shell_count(const int* N, const int* n, int ng) {
+  return 2LL * ng * ((long long)N[1] * N[2] + (long long)n[0] * N[2] + (long long)n[0] * n[1]);
+}
+
+// numpy's pairwise summation (loops_utils pairwise_sum) of elem(0..n-1), written
+// iteratively: runs of <= 128 are summed with 8 interleaved accumulators and longer
+// runs are split at n/2 rounded down to a multiple of 8.
+template <class F>
+__host__ __device__ double np_pairwise_leaf(const F& elem, int off, int n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r = r + elem(off + i);
+    return r;
+  }
+  double r[8];
+  for (int u = 0; u < 8; ++u) r[u] = elem(off + u);
+  int i = 8;
+  for (; i < n - (n % 8); i += 8)
+    for (int u = 0; u < 8; ++u) r[u] = r[u] + elem(off + i + u);
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res = res + elem(off + i);
+  return res;
+}
+
+template <class F>
+__host__ __device__ double np_pairwise(const F& elem, int n) {
+  int w_off[64], w_n[64], w_t[64];
+  double vals[40];
+  int nw = 0, nv = 0;
+  w_off[0] = 0; w_n[0] = n; w_t[0] = 0; nw = 1;
+  while (nw > 0) {
+    --nw;
+    const int off = w_off[nw], len = w_n[nw], t = w_t[nw];
+    if (t == 1) {
+      const double b = vals[--nv];
+      const double a = vals[--nv];
+      vals[nv++] = a + b;
+    } else if (len <= 128) {
+      vals[nv++] = np_pairwise_leaf(elem, off, len);
+    } else {
+      int h = len / 2;
+      h -= h % 8;
+      w_t[nw] = 1; w_off[nw] = 0; w_n[nw] = 0; ++nw;
+      w_t[nw] = 0; w_off[nw] = off + h; w_n[nw] = len - h; ++nw;
+      w_t[nw] = 0; w_off[nw] = off; w_n[nw] = h; ++nw;
+    }
+  }
+  return vals[0];
+}
+
+// numpy add.reduce of a run: the first element, then the pairwise sum of the rest
+template <class F>
+__host__ __device__ double np_sum(const F& elem, int n) {
+  if (n <= 0) return 0.0;
+  struct Shift {
+    const F& e;
+    __host__ __device__ double operator()(int i) const { return e(i + 1); }
+  } rest{elem};
+  return elem(0) + np_pairwise(rest, n - 1);
+}
+
+}  // namespace wmx

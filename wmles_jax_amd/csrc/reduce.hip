@@ -1,0 +1,234 @@
+// Whole-domain reductions of the step: CFL time step (solver.py:135-154), the
+// constant-mass-flux constraint (solver.py:16-36) and the sponge's spanwise mean
+// (smooth_ramp.py:214-216).  Two-level: per-workgroup partials (wavefront shuffles
+// + LDS), then one workgroup per ensemble case.  Built with -ffp-contract=off.
+#include "common.h"
+
+namespace wmx {
+
+__device__ __forceinline__ int find_block_r(const int* __restrict__ prefix, int nblk, int wg) {
+  int b = 0;
+  while (b + 1 < nblk && prefix[b + 1] <= wg) ++b;
+  return b;
+}
+
+// NaN-propagating min (numpy.min semantics)
+__device__ __forceinline__ double nmin(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a < b ? a : b;
+}
+
+template <bool MIN>
+__device__ double block_reduce(double v) {
+  __shared__ double red[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    double u = __shfl_xor(v, o, 64);
+    v = MIN ? nmin(v, u) : v + u;
+  }
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  double r = red[0];
+  for (int w = 1; w < nw; ++w) r = MIN ? nmin(r, red[w]) : r + red[w];
+  __syncthreads();
+  return r;
+}
+
+// interior cell l of block B -> ghosted linear index
+__device__ __forceinline__ long long interior_cell(const BlockDev& B, long long l, int* I, int* J,
+                                                   int* K) {
+  const int n1 = B.n[1], n2 = B.n[2], ng = B.ng;
+  *K = ng + (int)(l % n2);
+  *J = ng + (int)((l / n2) % n1);
+  *I = ng + (int)(l / ((long long)n1 * n2));
+  return cidx(B.N, *I, *J, *K);
+}
+
+// Per-cell CFL time step, min over interiors; solver.py:139-150.
+__global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__ blks,
+                                                    const int* __restrict__ wg_prefix, int nblk,
+                                                    Ptrs q, NumDev nd, double* __restrict__ part) {
+  const int b = find_block_r(wg_prefix, nblk, blockIdx.x);
+  const BlockDev& B = blks[b];
+  const long long l = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const long long nint = (long long)B.n[0] * B.n[1] * B.n[2];
+  double dtc = __builtin_huge_val();
+  if (l < nint) {
+    int I, J, K;
+    const long long c = interior_cell(B, l, &I, &J, &K);
+    const long long cells = B.cells;
+    const double* qb = q.p[b];
+    const GasC& gas = nd.gas;
+    Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
+                          qb[4 * cells + c]);
+    const double cs = sqrt(gas.gamma * gas.R * w.T);
+    const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
+    const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
+    const double vol = B.vol[c];
+    double lam = 0.0, lamv = 0.0;
+    const int cc[3] = {I, J, K};
+    for (int d = 0; d < 3; ++d) {
+      const double* S = B.S[d];
+      const long long fc = fcount(B.N, d);
+      int hi[3] = {cc[0], cc[1], cc[2]};
+      hi[d] += 1;
+      const long long f0 = fidx(B.N, d, cc[0], cc[1], cc[2]);
+      const long long f1 = fidx(B.N, d, hi[0], hi[1], hi[2]);
+      const double s0 = 0.5 * (S[f0] + S[f1]);
+      const double s1 = 0.5 * (S[fc + f0] + S[fc + f1]);
+      const double s2 = 0.5 * (S[2 * fc + f0] + S[2 * fc + f1]);
+      const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
+      lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
+      lamv = lamv + (nu * (Sm * Sm)) / vol;
+    }
+    dtc = nd.cfl / ((lam + 4.0 * lamv) / vol);
+  }
+  const double r = block_reduce<true>(dtc);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// sum over interiors of rho u_dir vol (and vol); solver.py:23-27
+__global__ __launch_bounds__(256) void k_mf_partial(const BlockDev* __restrict__ blks,
+                                                    const int* __restrict__ wg_prefix, int nblk,
+                                                    Ptrs q, const int* __restrict__ case_dir,
+                                                    double* __restrict__ part) {
+  const int b = find_block_r(wg_prefix, nblk, blockIdx.x);
+  const BlockDev& B = blks[b];
+  const long long l = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const long long nint = (long long)B.n[0] * B.n[1] * B.n[2];
+  const int dir = case_dir[B.case_id];
+  double num = 0.0, den = 0.0;
+  if (l < nint && dir >= 0) {
+    int I, J, K;
+    const long long c = interior_cell(B, l, &I, &J, &K);
+    const double vol = B.vol[c];
+    num = q.p[b][(1 + dir) * B.cells + c] * vol;
+    den = vol;
+  }
+  const double rn = block_reduce<false>(num);
+  const double rd = block_reduce<false>(den);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = rn;
+    part[2 * blockIdx.x + 1] = rd;
+  }
+}
+
+// One workgroup per case: combine the partials of the case's workgroups.
+// MODE 0: dt = min;  MODE 1: delta = target - num / den
+template <int MODE>
+__global__ void k_case_final(const int* __restrict__ case_wg, const int* __restrict__ case_wg_off,
+                             const double* __restrict__ part, const double* __restrict__ target,
+                             double* __restrict__ outv) {
+  const int cs = blockIdx.x;
+  const int lo = case_wg_off[cs], hi = case_wg_off[cs + 1];
+  double a = MODE == 0 ? __builtin_huge_val() : 0.0, b = 0.0;
+  for (int t = lo + threadIdx.x; t < hi; t += blockDim.x) {
+    const int wg = case_wg[t];
+    if (MODE == 0) a = nmin(a, part[wg]);
+    else { a += part[2 * wg]; b += part[2 * wg + 1]; }
+  }
+  if (MODE == 0) {
+    a = block_reduce<true>(a);
+    if (threadIdx.x == 0) outv[cs] = a;
+  } else {
+    a = block_reduce<false>(a);
+    b = block_reduce<false>(b);
+    if (threadIdx.x == 0) outv[cs] = target[cs] - a / b;
+  }
+}
+
+template __global__ void k_case_final<0>(const int*, const int*, const double*, const double*, double*);
+template __global__ void k_case_final<1>(const int*, const int*, const double*, const double*, double*);
+
+// q_dir += delta; rho E += (2 (rho u_dir) delta + delta^2) / (2 rho); all cells (solver.py:28-36)
+__global__ __launch_bounds__(256) void k_mf_apply(const BlockDev* __restrict__ blks,
+                                                  const int* __restrict__ wg_prefix, int nblk,
+                                                  Ptrs q, const int* __restrict__ case_dir,
+                                                  const double* __restrict__ delta) {
+  const int b = find_block_r(wg_prefix, nblk, blockIdx.x);
+  const BlockDev& B = blks[b];
+  const long long c = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  if (c >= B.cells) return;
+  const int dir = case_dir[B.case_id];
+  if (dir < 0) return;
+  const double dl = delta[B.case_id];
+  double* qb = q.p[b];
+  const long long cells = B.cells;
+  const double qm = qb[(1 + dir) * cells + c];
+  const double q0 = qb[c];
+  const double dE = ((2.0 * qm) * dl + dl * dl) / (2.0 * q0);
+  qb[c] = q0 + 0.0;
+  for (int x = 0; x < 3; ++x) {
+    const double v = qb[(1 + x) * cells + c];
+    qb[(1 + x) * cells + c] = v + (x == dir ? 0.0 + dl : 0.0);
+  }
+  qb[4 * cells + c] = qb[4 * cells + c] + dE;
+}
+
+// Volume-weighted spanwise (k) mean of the interior state per (i, j) row
+// (smooth_ramp.py:214-216), reduced in numpy's add.reduce order.  One thread per row.
+__global__ void k_sponge_mean(const BlockDev* __restrict__ blks, int blk, Ptrs q) {
+  const BlockDev& B = blks[blk];
+  long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long rows = (long long)B.n[0] * B.n[1];
+  if (row >= rows) return;
+  const int i = (int)(row / B.n[1]), j = (int)(row % B.n[1]);
+  const int ng = B.ng, nk = B.n[2];
+  const long long c0 = cidx(B.N, ng + i, ng + j, ng);
+  const double* vol = B.vol + c0;
+  const double den = np_sum([&](int k) { return vol[k]; }, nk);
+  const double* qb = q.p[blk];
+  for (int v = 0; v < 5; ++v) {
+    const double* qv = qb + v * B.cells + c0;
+    const double num = np_sum([&](int k) { return qv[k] * vol[k]; }, nk);
+    B.sponge_mean[v * rows + row] = num / den;
+  }
+}
+
+// ----------------------------------------------------------------------- launchers
+hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                             const NumDev& nd, double* d_part, hipStream_t s) {
+  if (nwg <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dt_partial, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_partial(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                             const int* d_case_dir, double* d_part, hipStream_t s) {
+  if (nwg <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mf_partial, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, d_case_dir,
+                     d_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_case_final(int mode, int ncases, const int* d_case_wg, const int* d_case_off,
+                             const double* d_part, const double* d_target, double* d_out,
+                             hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL(k_case_final<0>, dim3(ncases), dim3(256), 0, s, d_case_wg, d_case_off, d_part,
+                       d_target, d_out);
+  else
+    hipLaunchKernelGGL(k_case_final<1>, dim3(ncases), dim3(256), 0, s, d_case_wg, d_case_off, d_part,
+                       d_target, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_apply(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                           const int* d_case_dir, const double* d_delta, hipStream_t s) {
+  if (nwg <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mf_apply, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, d_case_dir,
+                     d_delta);
+  return hipGetLastError();
+}
+
+hipError_t launch_sponge_mean(const BlockDev* d_blks, int blk, long long rows, Ptrs q,
+                              hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sponge_mean, dim3((unsigned)((rows + 127) / 128)), dim3(128), 0, s, d_blks,
+                     blk, q);
+  return hipGetLastError();
+}
+
+}  // namespace wmx

@@ -1,0 +1,281 @@
+"""Multiblock explicit finite-volume driver on MI355X (reference `wmles/solver.py`).
+
+`Solver(mesh, gas, scheme, sgs="vreman", cfl=0.8, sgs_kwargs=None, constraints=(),
+sources=(), bc_passes=2)` with `init_state`, `apply_bcs`, `rhs`, `max_dt`,
+`advance`, `wall_stats` as in the reference.  State = tuple of float64 torch
+tensors (5, Ni, Nj, Nk) on the HIP device, ghosts included.  Every call returns new
+tensors and leaves its inputs unchanged.  All step arithmetic runs in the HIP
+kernels of libwmles_amd.so; `max_dt` returns a 0-d device tensor that `advance`
+consumes without a host round trip.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .backend import asarray, device
+from .bc import WallModel
+from .mesh import block_desc
+from .schemes import KEP, KEEP, MUSCLHLLC, Hybrid
+from .sgs import SGSModel, make_sgs
+
+
+class ConstantMassFlux:
+    """Post-step constraint: shift the volume-averaged rho u_dir of all block
+    interiors to `target`, keeping the internal energy (solver.py:16-36).  Inside
+    `Solver.advance` it runs as HIP reductions; called directly it applies the same
+    kernels through a one-off plan of the solver it is attached to."""
+
+    def __init__(self, direction, target):
+        if direction not in (0, 1, 2):
+            raise ValueError("direction must be 0, 1 or 2")
+        self.dir, self.target = direction, float(target)
+        self._solver = None
+
+    def __call__(self, Q, geoms, blocks):
+        if self._solver is None:
+            raise RuntimeError("ConstantMassFlux is applied by the Solver it was passed to")
+        out = tuple(q.clone() for q in self._solver._as_state(Q))
+        _lib.check(_lib.lib.wmx_mass_flux(self._solver._plan, _lib.ptr_array(out), _lib.stream_ptr()))
+        return out
+
+
+class _Ctx:
+    """Per-solver context handed to the face descriptors (inflow table ids)."""
+
+    def __init__(self):
+        self.inflows = []
+
+
+def numerics_desc(gas, scheme, sgs, cfl, bc_passes, constraints, sources, n_cases=1,
+                  precision=_lib.PREC_FP64):
+    nd = _lib.NumericsDesc()
+    nd.gamma, nd.R, nd.Pr, nd.Pr_t = gas.gamma, gas.R, gas.Pr, gas.Pr_t
+    nd.mu_ref, nd.T_ref, nd.S_suth = gas.mu_ref, gas.T_ref, gas.S_suth
+    nd.scheme, nd.central, nd.order, nd.limiter = _lib.SCHEME_KEP, _lib.SCHEME_KEP, 2, 0
+    nd.th0, nd.th1 = 0.6, 0.9
+    if scheme is not None:
+        nd.scheme = scheme.code
+        if isinstance(scheme, (KEP, KEEP)):
+            nd.order = scheme.order
+        elif isinstance(scheme, MUSCLHLLC):
+            nd.limiter = 0 if scheme.limiter == "albada" else 1
+        elif isinstance(scheme, Hybrid):
+            nd.central = scheme.c.code
+            nd.order = scheme.c.order
+            nd.limiter = 0 if scheme.u.limiter == "albada" else 1
+            nd.th0, nd.th1 = float(scheme.th0), float(scheme.th1)
+        else:
+            raise NotImplementedError(f"scheme {type(scheme).__name__} has no HIP implementation")
+    sg = sgs if isinstance(sgs, SGSModel) else make_sgs(None)
+    nd.sgs, nd.sgs_c = sg.code, sg.coef
+    from .wall_model import EquilibriumWallModel
+    wm = EquilibriumWallModel()
+    nd.kappa, nd.B, nd.n_iter = wm.kappa, wm.B, wm.n_iter
+    nd.wm_ekb, nd.wm_pr13, nd.wm_pr3, nd.wm_beta = wm.host_constants(gas)
+    nd.bc_passes = int(bc_passes)
+    nd.cfl = float(cfl)
+    nd.precision = precision
+    nd.n_sponge = 0
+    for src in sources:
+        from .smooth_ramp import SmoothRampSponge
+        if not isinstance(src, SmoothRampSponge):
+            raise NotImplementedError(f"source {type(src).__name__} has no HIP implementation")
+        if nd.n_sponge >= 4:
+            raise NotImplementedError("at most 4 sponge sources")
+        vals = (src.x_start_l, src.x_end_l, src.strength, src.props["U_inf"], src.props["L"])
+        for i, v in enumerate(vals):
+            nd.sponge[nd.n_sponge][i] = float(v)
+        nd.n_sponge += 1
+    nd.n_cases = n_cases
+    for c in range(16):
+        nd.mass_flux_dir[c] = -1
+        nd.mass_flux_target[c] = 0.0
+    for c, cons in enumerate(constraints):
+        nd.mass_flux_dir[c] = cons.dir
+        nd.mass_flux_target[c] = cons.target
+    return nd
+
+
+class Solver:
+    """solver.py:39-232."""
+
+    def __init__(self, mesh, gas, scheme, sgs="vreman", cfl=0.8, sgs_kwargs=None,
+                 constraints=(), sources=(), bc_passes=2):
+        self.mesh, self.blocks, self.gas = mesh, mesh.blocks, gas
+        self.geoms = mesh.geoms
+        if self.geoms is None:
+            raise RuntimeError("call Mesh.finalize() before building the Solver")
+        self.scheme, self.cfl = scheme, cfl
+        self.sgs = make_sgs(sgs, **(sgs_kwargs or {}))
+        self.constraints, self.sources, self.bc_passes = tuple(constraints), tuple(sources), bc_passes
+        native = [c for c in self.constraints if isinstance(c, ConstantMassFlux)]
+        if len(native) > 1 or (native and self.constraints[0] is not native[0]):
+            raise NotImplementedError("one ConstantMassFlux, first in the constraint list, is supported")
+        self._native_constraints = native
+        self._py_constraints = [c for c in self.constraints if not isinstance(c, ConstantMassFlux)]
+        for c in native:
+            c._solver = self
+        self._dev = device()
+        self._ctx = _Ctx()
+        self._keep = []
+        descs = [block_desc(b, mesh.index, self._ctx, g) for b, g in zip(self.blocks, self.geoms)]
+        self._descs = (_lib.BlockDesc * len(descs))(*descs)
+        self._nd = numerics_desc(gas, scheme, self.sgs, cfl, bc_passes, native, self.sources)
+        inflows = (_lib.InflowDesc * max(1, len(self._ctx.inflows)))()
+        for i, inf in enumerate(self._ctx.inflows):
+            inflows[i] = inf.desc(self._dev)
+            self._keep.append(inf)
+        plan = C.c_void_p()
+        _lib.check(_lib.lib.wmx_plan_create(len(descs), self._descs, C.byref(self._nd),
+                                             len(self._ctx.inflows), inflows, C.byref(plan)))
+        self._plan = plan
+        nbytes = C.c_size_t()
+        _lib.check(_lib.lib.wmx_plan_workspace_bytes(plan, C.byref(nbytes)))
+        self._ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=self._dev)
+        _lib.check(_lib.lib.wmx_plan_bind_workspace(plan, C.c_void_p(self._ws.data_ptr()), nbytes.value))
+        cnt = C.c_int32()
+        _lib.check(_lib.lib.wmx_plan_wall_faces(plan, C.byref(cnt), None))
+        rec = (C.c_int32 * (5 * max(1, cnt.value)))()
+        _lib.check(_lib.lib.wmx_plan_wall_faces(plan, C.byref(cnt), rec))
+        self._wall_faces = [tuple(rec[5 * i:5 * i + 5]) for i in range(cnt.value)]
+        self._dt_buf = torch.empty(16, dtype=torch.float64, device=self._dev)
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan:
+            try:
+                torch.cuda.synchronize()
+                _lib.lib.wmx_plan_destroy(plan)
+            except Exception:
+                pass
+
+    # ------------------------------------------------------------------ helpers
+    def _as_state(self, Q):
+        out = []
+        for q, b in zip(Q, self.blocks):
+            t = asarray(q)
+            if tuple(t.shape) != (5,) + tuple(b.N):
+                raise ValueError(f"block {b.name}: state must be (5,)+{b.N}, got {tuple(t.shape)}")
+            out.append(t)
+        if len(out) != len(self.blocks):
+            raise ValueError("one state array per block is required")
+        return tuple(out)
+
+    def fill_phases(self):
+        n = C.c_int32()
+        _lib.check(_lib.lib.wmx_plan_fill_phases(self._plan, C.byref(n)))
+        return n.value
+
+    # ------------------------------------------------------------------ init
+    def init_state(self, prim_fn):
+        """prim_fn(xc) -> [rho, u, v, w, p] at every cell centre, ghosts included
+        (solver.py:54-56).  xc is passed as a host NumPy array."""
+        out = []
+        for g in self.geoms:
+            w = prim_fn(g.xc.cpu().numpy())
+            w = asarray(np.asarray(w) if not isinstance(w, torch.Tensor) else w)
+            out.append(self.gas.prim_to_cons(w).contiguous())
+        return tuple(out)
+
+    # ------------------------------------------------------------------ step API
+    def apply_bcs(self, Q, geoms=None, time=0.0):
+        """Sequential ghost sweep (solver.py:59-73); returns new tensors."""
+        out = tuple(q.clone() for q in self._as_state(Q))
+        _lib.check(_lib.lib.wmx_apply_bcs(self._plan, _lib.ptr_array(out), float(time), _lib.stream_ptr()))
+        return out
+
+    def rhs(self, Q, time=0.0):
+        """RHS with zero ghosts (solver.py:97-106,159-160)."""
+        work = tuple(q.clone() for q in self._as_state(Q))
+        R = tuple(torch.empty_like(q) for q in work)
+        _lib.check(_lib.lib.wmx_rhs(self._plan, _lib.ptr_array(work), _lib.ptr_array(R), float(time),
+                                    _lib.stream_ptr()))
+        return R
+
+    def max_dt(self, Q, time=0.0):
+        """CFL time step (solver.py:135-154) as a 0-d device tensor."""
+        Q = self._as_state(Q)
+        out = torch.empty(16, dtype=torch.float64, device=self._dev)
+        _lib.check(_lib.lib.wmx_max_dt(self._plan, _lib.ptr_array(Q), C.c_void_p(out.data_ptr()),
+                                       _lib.stream_ptr()))
+        return out[0]
+
+    def advance(self, Q, dt, time=0.0):
+        """One SSP-RK3 step + constraints (solver.py:123-133,156-157)."""
+        Q = self._as_state(Q)
+        out = tuple(torch.empty_like(q) for q in Q)
+        if isinstance(dt, torch.Tensor):
+            d = dt.reshape(-1).to(device=self._dev, dtype=torch.float64)
+            if not d.is_contiguous():
+                d = d.contiguous()
+            rc = _lib.lib.wmx_advance(self._plan, _lib.ptr_array(Q), _lib.ptr_array(out), 0.0,
+                                      C.c_void_p(d.data_ptr()), float(time), _lib.stream_ptr())
+        else:
+            rc = _lib.lib.wmx_advance(self._plan, _lib.ptr_array(Q), _lib.ptr_array(out), float(dt),
+                                      None, float(time), _lib.stream_ptr())
+        _lib.check(rc)
+        for c in self._py_constraints:
+            out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
+        return out
+
+    # ------------------------------------------------------------------ diagnostics
+    def wall_stats(self, Q):
+        """{(block, face): (tau_w, q_w)} for all wall-modelled faces (solver.py:163-179)."""
+        Q = self._as_state(Q)
+        tot = sum(r[3] * r[4] for r in self._wall_faces)
+        tau = torch.empty(max(tot, 1), dtype=torch.float64, device=self._dev)
+        qw = torch.empty_like(tau)
+        _lib.check(_lib.lib.wmx_wall_stats(self._plan, _lib.ptr_array(Q), C.c_void_p(tau.data_ptr()),
+                                           C.c_void_p(qw.data_ptr()), _lib.stream_ptr()))
+        tau, qw = tau.cpu().numpy(), qw.cpu().numpy()
+        out, off = {}, 0
+        for b, a, s, n1, n2 in self._wall_faces:
+            cnt = n1 * n2
+            key = (self.blocks[b].name, "ijk"[a] + ("min", "max")[s])
+            out[key] = (tau[off:off + cnt].reshape(n1, n2), qw[off:off + cnt].reshape(n1, n2))
+            off += cnt
+        return out
+
+    def wall_loads(self, Q, moment_center=(0.0, 0.0, 0.0)):
+        """Integrated fluid-on-wall force and moment per WallModel face
+        (solver.py:181-232): tau_w from the HIP wall kernel, integration on the device."""
+        center = np.asarray(moment_center, dtype=float)
+        if center.shape != (3,) or not np.all(np.isfinite(center)):
+            raise ValueError("moment_center must contain three finite coordinates")
+        Q = self._as_state(Q)
+        stats = self.wall_stats(Q)
+        out = {}
+        cvec = torch.as_tensor(center, dtype=torch.float64, device=self._dev)
+        for blk, q, g in zip(self.blocks, Q, self.geoms):
+            w, T = self.gas.cons_to_prim(q)
+            for (axis, side), bc in blk.faces.items():
+                if not isinstance(bc, WallModel):
+                    continue
+                ng, n = blk.ng, blk.n[axis]
+                fi = ng if side == 0 else ng + n
+                mi = ng + bc.n_match - 1 if side == 0 else ng + n - bc.n_match
+                ci = ng if side == 0 else ng + n - 1
+                Sv = blk.take(g.S[axis], axis, fi)
+                area = torch.sqrt((Sv[0] * Sv[0] + Sv[1] * Sv[1]) + Sv[2] * Sv[2])
+                nrm = Sv / area
+                if side == 0:
+                    nrm = -nrm
+                vel = blk.take(w[1:4], axis, mi)
+                un = (vel[0] * nrm[0] + vel[1] * nrm[1]) + vel[2] * nrm[2]
+                ut = vel - un[None] * nrm
+                speed = torch.sqrt(((ut[0] ** 2 + ut[1] ** 2) + ut[2] ** 2) + 1e-20)
+                tang = ut / speed
+                face = "ijk"[axis] + ("min", "max")[side]
+                tau = torch.as_tensor(stats[(blk.name, face)][0], device=self._dev)
+                pw = blk.take(w[4], axis, ci)
+                traction = (pw[None] * nrm + tau[None] * tang) * area[None]
+                fc = blk.take(g.xf[axis], axis, fi)
+                arm = fc - cvec.reshape(3, 1, 1)
+                mom = torch.stack([arm[1] * traction[2] - arm[2] * traction[1],
+                                   arm[2] * traction[0] - arm[0] * traction[2],
+                                   arm[0] * traction[1] - arm[1] * traction[0]])
+                out[(blk.name, face)] = (traction.sum(dim=(1, 2)).cpu().numpy(),
+                                         mom.sum(dim=(1, 2)).cpu().numpy())
+        return out
