@@ -139,6 +139,15 @@ int wmx_plan_wall_faces(const wmx_plan* plan, int32_t* count, int32_t* records /
 /* Ghost-fill phases after dependency grouping (diagnostic). */
 int wmx_plan_fill_phases(const wmx_plan* plan, int32_t* nphases);
 
+/* Kernel-class event timing (measurement aid, no reference counterpart).  Classes:
+   0 flux+RK, 1 gradient/SGS, 2 primitives, 3 ghost fill, 4 wall model, 5 CFL dt,
+   6 mass flux, 7 ghost shell.  wmx_plan_profile(plan, mask) brackets every later
+   launch of a class whose bit is set in `mask` with hipEvents on the launch stream
+   (mask 0 stops and clears); wmx_plan_profile_read synchronises the recorded events
+   and returns per-class summed milliseconds and launch counts (8 entries each). */
+int wmx_plan_profile(wmx_plan* plan, int32_t mask);
+int wmx_plan_profile_read(wmx_plan* plan, double* ms, int32_t* counts);
+
 /* Setup: ghost nodes by node exchange / point reflection, `node_passes` sweeps.
    X[b]: device (3, Ni+1, Nj+1, Nk+1) with interior nodes set.
    Replaces wmles/mesh.py:213-232 (Mesh.finalize ghost-node loop, _reflect_nodes). */

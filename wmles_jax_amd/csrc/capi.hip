@@ -97,11 +97,41 @@ struct wmx_plan {
   char* ws = nullptr;
   double* d_part = nullptr;
   double* d_delta = nullptr;
+  // kernel-class event timing
+  int prof_mask = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_rec;   // (class, index of start event)
 };
 
 namespace {
 
 int nwg_of(const std::vector<int>& pre) { return pre.empty() ? 0 : pre.back(); }
+
+enum { K_FLUX = 0, K_GRAD, K_PRIM, K_FILL, K_WALL, K_DT, K_MF, K_SHELL, K_NCLASS };
+
+hipEvent_t next_event(wmx_plan* p) {
+  if (p->ev_used == p->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    p->ev_pool.push_back(e);
+  }
+  return p->ev_pool[p->ev_used++];
+}
+
+// bracket one launch of class `cls` with events when profiling is on
+template <class F>
+hipError_t timed(wmx_plan* p, int cls, hipStream_t s, F&& launch) {
+  if (!(p->prof_mask & (1 << cls))) return launch();
+  const size_t i0 = p->ev_used;
+  hipEvent_t a = next_event(p), b = next_event(p);
+  if (!a || !b) return launch();
+  hipEventRecord(a, s);
+  hipError_t e = launch();
+  hipEventRecord(b, s);
+  p->ev_rec.push_back({cls, i0});
+  return e;
+}
 
 std::vector<int> prefix(const std::vector<long long>& counts) {
   std::vector<int> p(counts.size() + 1, 0);
@@ -223,7 +253,7 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
   const int nph = (int)p->phase_max.size();
   for (int ph = 0; ph < nph; ++ph) {
     const int o = p->phase_off[ph], n = p->phase_off[ph + 1] - o;
-    WMX_CHECK(launch_fill_cells(p->d_blks, p->d_ops + o, n, p->phase_max[ph], q, p->nd.gas, s));
+    WMX_CHECK(timed(p, K_FILL, s, [&] { return launch_fill_cells(p->d_blks, p->d_ops + o, n, p->phase_max[ph], q, p->nd.gas, s); }));
     for (int blk : p->phase_inflow[ph]) {
       const BlockDev& B = p->hb[blk];
       const int id = B.faces[0].inflow_id;
@@ -237,9 +267,9 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
 // ghost fill + cell-level fields of one RHS evaluation
 int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s) {
   if (run_fill(p, q, time, d_dt, coef, s)) return 1;
-  WMX_CHECK(launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd.gas, s));
-  WMX_CHECK(launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s));
-  WMX_CHECK(launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s));
+  WMX_CHECK(timed(p, K_PRIM, s, [&] { return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd.gas, s); }));
+  WMX_CHECK(timed(p, K_GRAD, s, [&] { return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s); }));
+  WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
   if (p->nd.n_sponge > 0)
     for (int b = 0; b < p->nb; ++b)
       WMX_CHECK(launch_sponge_mean(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], q, s));
@@ -421,6 +451,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
 
 int wmx_plan_destroy(wmx_plan* p) {
   if (!p) return 0;
+  for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   (void)hipFree(p->d_blks);
   (void)hipFree(p->d_ops);
   (void)hipFree(p->d_wall);
@@ -483,6 +514,28 @@ int wmx_plan_wall_faces(const wmx_plan* p, int32_t* count, int32_t* rec) {
 int wmx_plan_fill_phases(const wmx_plan* p, int32_t* nphases) {
   if (!p || !nphases) return fail("null argument");
   *nphases = (int32_t)p->phase_max.size();
+  return 0;
+}
+
+int wmx_plan_profile(wmx_plan* p, int32_t mask) {
+  if (!p) return fail("null plan");
+  p->prof_mask = mask;
+  p->ev_used = 0;
+  p->ev_rec.clear();
+  return 0;
+}
+
+int wmx_plan_profile_read(wmx_plan* p, double* ms, int32_t* counts) {
+  if (!p || !ms || !counts) return fail("null argument");
+  for (int c = 0; c < K_NCLASS; ++c) { ms[c] = 0.0; counts[c] = 0; }
+  for (auto& r : p->ev_rec) {
+    hipEvent_t a = p->ev_pool[r.second], b = p->ev_pool[r.second + 1];
+    WMX_CHECK(hipEventSynchronize(b));
+    float t = 0.f;
+    WMX_CHECK(hipEventElapsedTime(&t, a, b));
+    ms[r.first] += t;
+    counts[r.first] += 1;
+  }
   return 0;
 }
 
@@ -562,12 +615,17 @@ int wmx_mass_flux(wmx_plan* p, double* const* q, void* stream) {
   if (!p->any_constraint) return 0;
   hipStream_t s = as_stream(stream);
   Ptrs pq = make_ptrs(p, q);
-  WMX_CHECK(launch_mf_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq, p->d_case_dir,
-                              p->d_part, s));
-  WMX_CHECK(launch_case_final(1, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part,
-                              p->d_case_target, p->d_delta, s));
-  WMX_CHECK(launch_mf_apply(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), pq, p->d_case_dir,
-                            p->d_delta, s));
+  WMX_CHECK(timed(p, K_MF, s, [&] {
+    hipError_t e = launch_mf_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq,
+                                     p->d_case_dir, p->d_part, s);
+    if (e == hipSuccess)
+      e = launch_case_final(1, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part,
+                            p->d_case_target, p->d_delta, s);
+    if (e == hipSuccess)
+      e = launch_mf_apply(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), pq, p->d_case_dir,
+                          p->d_delta, s);
+    return e;
+  }));
   return 0;
 }
 
@@ -579,17 +637,18 @@ int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, 
   hipStream_t s = as_stream(stream);
   Ptrs P0 = make_ptrs(p, q0), PO = make_ptrs(p, qout);
   const int ntiles = (int)p->tiles.size();
-  WMX_CHECK(launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s));
+  WMX_CHECK(timed(p, K_SHELL, s, [&] { return launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s); }));
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
   if (run_cells(p, P0, time, d_dt, 0.0, s)) return 1;
-  WMX_CHECK(launch_flux(1, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, P0, P0, PO, d_dt, dt, s));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, P0, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
-  WMX_CHECK(launch_flux(2, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t2, d_dt, 0.5, s)) return 1;
-  WMX_CHECK(launch_flux(3, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s));
-  WMX_CHECK(launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO,
-                                d_dt, dt, s));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(3, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_SHELL, s, [&] {
+    return launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s);
+  }));
   return wmx_mass_flux(p, qout, stream);
 }
 
@@ -598,9 +657,13 @@ int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
   if (!d_dt) return fail("null d_dt");
   hipStream_t s = as_stream(stream);
   Ptrs pq = make_ptrs(p, q);
-  WMX_CHECK(launch_dt_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq, p->nd,
-                              p->d_part, s));
-  WMX_CHECK(launch_case_final(0, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part, nullptr, d_dt, s));
+  WMX_CHECK(timed(p, K_DT, s, [&] {
+    hipError_t e = launch_dt_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq, p->nd,
+                                     p->d_part, s);
+    if (e == hipSuccess)
+      e = launch_case_final(0, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part, nullptr, d_dt, s);
+    return e;
+  }));
   return 0;
 }
 

@@ -163,6 +163,22 @@ class Solver:
             raise ValueError("one state array per block is required")
         return tuple(out)
 
+    KERNEL_CLASSES = ("flux", "grad", "prim", "fill", "wall", "dt", "massflux", "shell")
+
+    def profile(self, classes=("flux",)):
+        """Start event timing of the named kernel classes (empty: stop)."""
+        mask = 0
+        for c in classes:
+            mask |= 1 << self.KERNEL_CLASSES.index(c)
+        _lib.check(_lib.lib.wmx_plan_profile(self._plan, mask))
+
+    def profile_read(self):
+        """{class: (total_ms, launches)} of the launches timed since profile()."""
+        ms = (C.c_double * 8)()
+        cnt = (C.c_int32 * 8)()
+        _lib.check(_lib.lib.wmx_plan_profile_read(self._plan, ms, cnt))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES) if cnt[i]}
+
     def fill_phases(self):
         n = C.c_int32()
         _lib.check(_lib.lib.wmx_plan_fill_phases(self._plan, C.byref(n)))
