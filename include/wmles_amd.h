@@ -138,6 +138,16 @@ int wmx_plan_bind_workspace(wmx_plan* plan, void* d_workspace, size_t bytes);
 int wmx_plan_wall_faces(const wmx_plan* plan, int32_t* count, int32_t* records /* 5*count or NULL */);
 /* Ghost-fill phases after dependency grouping (diagnostic). */
 int wmx_plan_fill_phases(const wmx_plan* plan, int32_t* nphases);
+/* Sweep passes executed: bc_passes minus the trailing passes shown (by a symbolic run
+   of the sweep at plan creation) to reproduce the previous pass bit for bit. */
+int wmx_plan_fill_passes(const wmx_plan* plan, int32_t* npasses);
+
+/* FP32-bulk gauge (no reference counterpart: precision form of this build).  The
+   FP32 records hold p - p_ref, T - T_ref and e_int - R T_ref / (gamma - 1); the
+   fluxes add the shift's contribution back in FP64.  Values near the flow's mean
+   pressure and temperature keep the low-Mach cancellation in FP64.  Ignored by the
+   FP64 plain form. */
+int wmx_plan_set_gauge(wmx_plan* plan, double p_ref, double T_ref);
 
 /* Kernel-class event timing (measurement aid, no reference counterpart).  Classes:
    0 flux+RK, 1 gradient/SGS, 2 primitives, 3 ghost fill, 4 wall model, 5 CFL dt,

@@ -24,8 +24,9 @@ How each function is pinned (a plausible mistake fails at least one check):
 * MUSCL-HLLC: Sod shock-tube plateau values against the exact Riemann solution;
 * viscous flux: exact stress / heat flux for linear fields on a Cartesian grid.
 
-`KEEP` (reference `wmles/schemes.py:25-55,72-98`) has no golden vector and is
-pinned only by the constant-state and free-stream checks.
+`KEEP` (reference `wmles/schemes.py:25-55,72-98`, not exported by the reference's
+`__init__`) is pinned by the golden case `single_keep4_wmadi` and by the
+constant-state and free-stream checks.
 """
 from .gas import Gas
 from .mesh import Block, Mesh, Interface, Geom, compute_geometry, exchange, FACE_NAMES, sl

@@ -68,6 +68,7 @@ struct wmx_plan {
   std::vector<int> case_dir;
   std::vector<double> case_target;
   // ghost sweep
+  int passes = 0;                     // sweep passes kept after the redundancy check
   std::vector<GhostOp> ops;           // all phases, flattened
   std::vector<int> phase_off;         // size nphases + 1
   std::vector<int> phase_max;
@@ -170,6 +171,79 @@ void op_boxes(const wmx_plan& p, int b, int a, int s, std::vector<Box>* rd, Box*
   }
 }
 
+// ---- redundant sweep passes ------------------------------------------------------
+// Executes the sweep symbolically on 64-bit value signatures: interior cells and the
+// initial ghost values get unique ids; an interface copy carries its source's
+// signature; a boundary fill hashes (op, destination layer position, source
+// signature).  Equal signatures mean equal expression trees, hence bit-identical
+// floating-point values.  Passes after the first one that leaves every signature
+// unchanged are dropped: they would rewrite the same values.
+inline uint64_t mix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+
+int effective_passes(const wmx_plan& p, int passes) {
+  if (passes <= 1) return passes;
+  const int nb = p.nb;
+  std::vector<std::vector<uint64_t>> sig(nb);
+  uint64_t next = 1;
+  for (int b = 0; b < nb; ++b) {
+    sig[b].resize((size_t)p.hb[b].cells);
+    for (auto& v : sig[b]) v = mix64(next++);
+  }
+  auto run_pass = [&]() {
+    int opi = 0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < nb; ++b)
+        for (int s = 0; s < 2; ++s, ++opi) {
+          const BlockDev& B = p.hb[b];
+          const wmx_face_desc& f = B.faces[2 * a + s];
+          const int ng = B.ng, n = B.n[a];
+          int t1, t2;
+          tangents(a, &t1, &t2);
+          for (int m = 0; m < ng; ++m)
+            for (int i1 = 0; i1 < B.N[t1]; ++i1)
+              for (int i2 = 0; i2 < B.N[t2]; ++i2) {
+                int c[3];
+                c[t1] = i1; c[t2] = i2;
+                c[a] = s == 1 ? ng + n + m : ng - 1 - m;
+                const size_t di = (size_t)cidx(B.N, c[0], c[1], c[2]);
+                uint64_t v;
+                if (f.kind == WMX_FACE_INTERFACE) {
+                  const BlockDev& S = p.hb[f.nb];
+                  int cs[3];
+                  for (int t = 0; t < 3; ++t) {
+                    if (t == a) continue;
+                    cs[f.mapping[t]] = f.flip[t] ? B.N[t] - 1 - c[t] : c[t];
+                  }
+                  const int nB = S.n[f.src_axis];
+                  cs[f.src_axis] = f.src_side == 1 ? nB + ng - 1 - m : ng + m;
+                  v = sig[f.nb][(size_t)cidx(S.N, cs[0], cs[1], cs[2])];
+                } else {
+                  int ci[3] = {c[0], c[1], c[2]};
+                  ci[a] = s == 0 ? ng + m : ng + n - 1 - m;
+                  if (f.kind == WMX_BC_EXTRAPOLATE) ci[a] = s == 0 ? ng : ng + n - 1;
+                  uint64_t src = 0;
+                  if (f.kind != WMX_BC_DIRICHLET && f.kind != WMX_BC_INFLOW)
+                    src = sig[b][(size_t)cidx(B.N, ci[0], ci[1], ci[2])];
+                  v = mix64(mix64(((uint64_t)opi << 40) ^ di) ^ src);
+                }
+                sig[b][di] = v;
+              }
+        }
+  };
+  run_pass();
+  for (int k = 1; k < passes; ++k) {
+    std::vector<std::vector<uint64_t>> prev = sig;
+    run_pass();
+    if (prev == sig) return k;          // pass k+1 reproduced pass k exactly
+  }
+  return passes;
+}
+
 int build_phases(wmx_plan& p, int passes) {
   struct Pending {
     GhostOp op;
@@ -239,6 +313,10 @@ NumDev make_numdev(const wmx_numerics_desc* nm) {
   nd.cfl = nm->cfl;
   nd.n_sponge = nm->n_sponge;
   std::memcpy(nd.sponge, nm->sponge, sizeof(nd.sponge));
+  nd.prec = nm->precision;
+  nd.p_ref = 0.0;
+  nd.T_gauge = 0.0;
+  nd.e_ref = 0.0;
   return nd;
 }
 
@@ -267,7 +345,7 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
 // ghost fill + cell-level fields of one RHS evaluation
 int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s) {
   if (run_fill(p, q, time, d_dt, coef, s)) return 1;
-  WMX_CHECK(timed(p, K_PRIM, s, [&] { return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd.gas, s); }));
+  WMX_CHECK(timed(p, K_PRIM, s, [&] { return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd, s); }));
   WMX_CHECK(timed(p, K_GRAD, s, [&] { return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s); }));
   WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
   if (p->nd.n_sponge > 0)
@@ -302,6 +380,8 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   if (nm->n_sponge < 0 || nm->n_sponge > 4) return fail("at most 4 sponge sources");
   if (nm->order != 2 && nm->order != 4) return fail("central order must be 2 or 4");
   if (nm->bc_passes < 1) return fail("bc_passes must be >= 1");
+  if (nm->precision != WMX_PREC_FP64 && nm->precision != WMX_PREC_FP32_BULK)
+    return fail("precision must be WMX_PREC_FP64 or WMX_PREC_FP32_BULK");
   wmx_plan* p = new wmx_plan();
   p->nb = nblocks;
   p->desc.assign(blocks, blocks + nblocks);
@@ -367,7 +447,8 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     p->inflows.push_back(InflowDev{in.n_tab, in.n_modes, in.table, in.modes, in.U_inf, in.L,
                                    in.rho_inf, in.p_inf});
   }
-  build_phases(*p, nm->bc_passes);
+  p->passes = effective_passes(*p, nm->bc_passes);
+  build_phases(*p, p->passes);
   // wall faces
   for (int b = 0; b < nblocks; ++b)
     for (int f = 0; f < 6; ++f)
@@ -409,10 +490,11 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   p->off_wall.assign(nblocks, std::vector<long long>(6, -1));
   for (int b = 0; b < nblocks; ++b) {
     const BlockDev& B = p->hb[b];
+    const size_t rec = nd.prec == WMX_PREC_FP32_BULK ? sizeof(float) : sizeof(double);
     p->off_P.push_back(off);
-    off = align_up(off + sizeof(double) * NP * B.cells);
+    off = align_up(off + rec * NP * B.cells);
     p->off_G.push_back(off);
-    off = align_up(off + sizeof(double) * NGR * B.cells);
+    off = align_up(off + rec * NGR * B.cells);
     p->off_shell.push_back(off);
     off = align_up(off + sizeof(double) * 5 * B.shell_cells);
     p->off_sponge.push_back(off);
@@ -483,6 +565,8 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
     BlockDev& B = p->hb[b];
     B.P = reinterpret_cast<double*>(p->ws + p->off_P[b]);
     B.G = reinterpret_cast<double*>(p->ws + p->off_G[b]);
+    B.Pf = reinterpret_cast<float*>(p->ws + p->off_P[b]);
+    B.Gf = reinterpret_cast<float*>(p->ws + p->off_G[b]);
     B.shell = reinterpret_cast<double*>(p->ws + p->off_shell[b]);
     B.sponge_mean = reinterpret_cast<double*>(p->ws + p->off_sponge[b]);
     for (int f = 0; f < 6; ++f)
@@ -514,6 +598,21 @@ int wmx_plan_wall_faces(const wmx_plan* p, int32_t* count, int32_t* rec) {
 int wmx_plan_fill_phases(const wmx_plan* p, int32_t* nphases) {
   if (!p || !nphases) return fail("null argument");
   *nphases = (int32_t)p->phase_max.size();
+  return 0;
+}
+
+int wmx_plan_fill_passes(const wmx_plan* p, int32_t* npasses) {
+  if (!p || !npasses) return fail("null argument");
+  *npasses = (int32_t)p->passes;
+  return 0;
+}
+
+int wmx_plan_set_gauge(wmx_plan* p, double p_ref, double T_ref) {
+  if (!p) return fail("null plan");
+  if (!(p_ref >= 0.0) || !(T_ref >= 0.0)) return fail("gauge values must be finite and >= 0");
+  p->nd.p_ref = p_ref;
+  p->nd.T_gauge = T_ref;
+  p->nd.e_ref = p->nd.gas.R * T_ref / (p->nd.gas.gamma - 1.0);
   return 0;
 }
 

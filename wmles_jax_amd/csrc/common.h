@@ -29,8 +29,10 @@ struct BlockDev {
   const double* xc;
   const double* S[3];
   const double* xf[3];
-  double* P;                // NP planes
-  double* G;                // NGR planes
+  double* P;                // NP planes (FP64 form)
+  double* G;                // NGR planes (FP64 form)
+  float* Pf;                // NP planes (FP32-bulk form; same workspace region as P)
+  float* Gf;                // NGR planes (FP32-bulk form)
   double* wall[6];          // WALL_PLANES planes of (n_t1, n_t2) per WallModel face
   double* sponge_mean;      // (5, n0, n1)
   double* shell;            // (5, shell cells) saved ghost values
@@ -48,6 +50,9 @@ struct NumDev {
   double cfl;
   int n_sponge;
   double sponge[4][5];
+  int prec;                 // WMX_PREC_FP64 / WMX_PREC_FP32_BULK
+  double p_ref, T_gauge, e_ref;   // FP32-bulk gauge: p' = p - p_ref, T' = T - T_gauge,
+                                  // e' = e_int - e_ref with e_ref = R T_gauge / (gamma - 1)
 };
 
 struct GhostOp {
@@ -77,8 +82,8 @@ hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng
                          hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);
-hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q, GasC gas,
-                       hipStream_t s);
+hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                       const NumDev& nd, hipStream_t s);
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
                        hipStream_t s);
 hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, int maxcount, Ptrs q,

@@ -1,0 +1,635 @@
+// Fused face fluxes + divergence + SSP-RK3 stage update (FP64 plain form).
+//
+// One workgroup owns a 16 x 16 (j, k) column tile and marches i over a chunk of
+// planes.  Each thread keeps the i-direction stencil of its own column in registers
+// (primitives at i-1..i+2, gradient records and centroids at i, i+1), so every plane
+// is read from HBM once.  The current plane is staged in LDS with the halo the j/k
+// faces need (primitives: 2 cells, gradient records and centroids: 1 cell).  Every
+// face flux is evaluated once: the i face of each column in registers, the j/k faces
+// of the tile (plus its upper edge rows) through LDS, and each cell sums its six faces
+// (reference solver.py:86-95: inviscid scheme + viscous flux, wall override,
+// divergence / vol, then sources, solver.py:103-104, and the stage combination,
+// solver.py:109-121).
+#include "common.h"
+
+namespace wmx {
+
+namespace {
+
+constexpr int TJ = 16, TK = 16, NT = TJ * TK;
+constexpr int HJ = TJ + 4, HK = TK + 4;      // primitive region (halo 2)
+constexpr int GJ = TJ + 2, GK = TK + 2;      // gradient / centroid region (halo 1)
+constexpr int NPW = 7;                       // r u v w p e T
+constexpr int NGW = 15;                      // g[12], mu_eff, k_eff, theta
+
+template <class R>
+struct ColT {                                // one cell's primitive record + T
+  CellWT<R> w;
+  R T;
+};
+
+template <class R>
+struct ColGT {
+  CellGT<R> g;
+  R th;
+};
+
+template <class R>
+__device__ __forceinline__ ColT<R> load_col(const R* __restrict__ P, long long cells, long long c) {
+  ColT<R> o;
+  o.w.r = P[c];
+  o.w.u = P[cells + c];
+  o.w.v = P[2 * cells + c];
+  o.w.w = P[3 * cells + c];
+  o.w.p = P[4 * cells + c];
+  o.w.e = P[P_E * cells + c];
+  o.T = P[P_T * cells + c];
+  return o;
+}
+
+template <class R>
+__device__ __forceinline__ ColGT<R> load_colg(const R* __restrict__ G, long long cells, long long c) {
+  ColGT<R> o;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) o.g.g[i] = G[i * cells + c];
+  o.g.mu = G[G_MU * cells + c];
+  o.g.k = G[G_K * cells + c];
+  o.th = G[G_TH * cells + c];
+  return o;
+}
+
+// Inviscid (scheme) + viscous flux through one face, both including the area, in the
+// bulk type R with the result in FP64.  For R = float the central part carries
+// p - p_ref and e_int - e_ref; their contribution p_ref S (momentum) and
+// e_ref m + p_ref u_bar.S (energy) is added back in FP64 with the FP64 face vector.
+// MUSCL-HLLC reconstructs the shifted pressure and adds p_ref before the Riemann
+// solve.  The cell-to-cell unit vector and distance come from FP64 centroids.
+template <class R>
+__device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2, const CellWT<R>& L,
+                                          const CellWT<R>& Rc, const CellWT<R>& R2, R thL, R thR,
+                                          const CellGT<R>& gL, const CellGT<R>& gR, R TL, R TR,
+                                          const double* xL, const double* xR, double Sx, double Sy,
+                                          double Sz, double* F) {
+  const R sx = R(Sx), sy = R(Sy), sz = R(Sz);
+  const int sch = nd.scheme;
+  const bool gauge = sizeof(R) == 4;
+  double Fc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Fu[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double s = sch == WMX_SCHEME_MUSCLHLLC ? 1.0 : 0.0;
+  if (sch == WMX_SCHEME_HYBRID) {
+    const R th = fmax(thL, thR);
+    s = (double)fmin(fmax((th - R(nd.th0)) / R(nd.th1 - nd.th0), R(0.0)), R(1.0));
+  }
+  if (s < 1.0) {
+    R Fr[5], un4;
+    const bool keep = sch == WMX_SCHEME_KEEP || (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP);
+    if (keep) central_flux<true, R>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fr, &un4);
+    else central_flux<false, R>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fr, &un4);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Fc[v] = (double)Fr[v];
+    if (gauge) {
+      Fc[1] += nd.p_ref * Sx;
+      Fc[2] += nd.p_ref * Sy;
+      Fc[3] += nd.p_ref * Sz;
+      Fc[4] += nd.e_ref * Fc[0] + nd.p_ref * (double)un4;
+    }
+  }
+  if (s > 0.0) {
+    R Fr[5];
+    muscl_hllc_flux<R>(R(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz, R(nd.p_ref), Fr);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Fu[v] = (double)Fr[v];
+  }
+  if (sch == WMX_SCHEME_HYBRID) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = (1.0 - s) * Fc[v] + s * Fu[v];
+  } else if (sch == WMX_SCHEME_MUSCLHLLC) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = Fu[v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = Fc[v];
+  }
+  const double dx0 = xR[0] - xL[0], dx1 = xR[1] - xL[1], dx2 = xR[2] - xL[2];
+  const double dist = sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
+  const R phL[4] = {L.u, L.v, L.w, TL};
+  const R phR[4] = {Rc.u, Rc.v, Rc.w, TR};
+  R Fv[5] = {R(0), R(0), R(0), R(0), R(0)};
+  viscous_face_e<R>(gL, gR, phL, phR, R(dx0 / dist), R(dx1 / dist), R(dx2 / dist), R(dist), sx, sy,
+                    sz, Fv);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) F[v] = F[v] + (double)Fv[v];
+}
+
+__device__ __forceinline__ void ring_slot(int t, int h, int* sj, int* sk) {
+  const int W = TK + 2 * h, top = h * W;
+  if (t < top) { *sj = t / W; *sk = t % W; return; }
+  if (t < 2 * top) { t -= top; *sj = TJ + h + t / W; *sk = t % W; return; }
+  t -= 2 * top;
+  const int row = t / (2 * h), c = t % (2 * h);
+  *sj = h + row;
+  *sk = c < h ? c : TK + c;
+}
+
+template <class R>
+__device__ __forceinline__ CellWT<R> sp_w(const R (*sP)[HJ * HK], int s) {
+  CellWT<R> w;
+  w.r = sP[0][s]; w.u = sP[1][s]; w.v = sP[2][s]; w.w = sP[3][s]; w.p = sP[4][s]; w.e = sP[5][s];
+  return w;
+}
+
+template <class R>
+__device__ __forceinline__ CellGT<R> sg_g(const R (*sG)[GJ * GK], int s) {
+  CellGT<R> g;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) g.g[i] = sG[i][s];
+  g.mu = sG[12][s];
+  g.k = sG[13][s];
+  return g;
+}
+
+}  // namespace
+
+// MODE 0: r = RHS; 1: q + dt r; 2: 3/4 q0 + 1/4 (q + dt r); 3: 1/3 q0 + 2/3 (q + dt r)
+template <class R, int MODE>
+__global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ blks,
+                                               const int4* __restrict__ tiles, int ci_len,
+                                               NumDev nd, Ptrs qin, Ptrs q0, Ptrs out,
+                                               const double* __restrict__ d_dt, double dt_host) {
+  // staging area: primitives (halo 2) and gradient records (halo 1) in R; the FP64
+  // face fluxes of the plane go to their own exchange buffers as they are computed
+  __shared__ R sP[NPW][HJ * HK];
+  __shared__ R sG[NGW][GJ * GK];
+  __shared__ double sX[3][GJ * GK];
+  __shared__ double sFj[5][(TJ + 1) * TK];
+  __shared__ double sFk[5][TJ * (TK + 1)];
+  const int4 t = tiles[blockIdx.x];
+  const BlockDev& B = blks[t.x];
+  const int j0 = t.y, k0 = t.z, i0 = t.w;
+  const int i1 = min(i0 + ci_len, B.n[0]);
+  const int tid = threadIdx.x;
+  const int tj = tid / TK, tk = tid % TK;
+  const int ng = B.ng;
+  const int* N = B.N;
+  const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
+  const int j = j0 + tj, k = k0 + tk;
+  const bool act = j < n1 && k < n2;
+  const int jl = min(j0 + TJ, n1) - j0;
+  const int kl = min(k0 + TK, n2) - k0;
+  const int J = ng + j, K = ng + k;
+  const long long cells = B.cells;
+  const long long sI = (long long)N[1] * N[2];
+  const double dt = d_dt ? d_dt[B.case_id] : dt_host;
+  const R* __restrict__ P = sizeof(R) == 8 ? (const R*)B.P : (const R*)B.Pf;
+  const R* __restrict__ G = sizeof(R) == 8 ? (const R*)B.G : (const R*)B.Gf;
+  const double* __restrict__ XC = B.xc;
+  // boxes the staged data may come from: primitives [0, N), gradients [ng-1, ng+n]
+  auto p_ok = [&](int jj, int kk) { return jj >= 0 && jj < N[1] && kk >= 0 && kk < N[2]; };
+  auto g_ok = [&](int jj, int kk) {
+    return jj >= ng - 1 && jj <= ng + n1 && kk >= ng - 1 && kk <= ng + n2;
+  };
+  // i-direction register window of the own column
+  ColT<R> Pw[4];
+  ColGT<R> Gw[2];
+  double Xw[2][3];
+  const int I0 = ng + i0;
+  if (act) {
+    const long long cb = (long long)J * N[2] + K;
+#pragma unroll
+    for (int s = 1; s < 4; ++s) Pw[s] = load_col<R>(P, cells, (I0 - 3 + s) * sI + cb);
+    Gw[1] = load_colg<R>(G, cells, (I0 - 1) * sI + cb);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I0 - 1) * sI + cb];
+  }
+  double Flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Fhi[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int I = I0 - 1; I < ng + i1; ++I) {
+    const bool first = I == I0 - 1;
+    const long long cb = (long long)J * N[2] + K;
+    // ---- advance the window: load plane I+2 (primitives), I+1 (gradients, centroids)
+    if (act) {
+      Pw[0] = Pw[1]; Pw[1] = Pw[2]; Pw[2] = Pw[3];
+      Pw[3] = load_col<R>(P, cells, (I + 2) * sI + cb);
+      Gw[0] = Gw[1];
+      Gw[1] = load_colg<R>(G, cells, (I + 1) * sI + cb);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        Xw[0][x] = Xw[1][x];
+        Xw[1][x] = XC[x * cells + (I + 1) * sI + cb];
+      }
+    }
+    // ---- stage plane I in LDS: own columns from registers, halo and idle columns from HBM
+    if (!first) {
+      const int sp = (tj + 2) * HK + (tk + 2), sg = (tj + 1) * GK + (tk + 1);
+      const long long pl = (long long)I * sI;
+      if (act) {
+        sP[0][sp] = Pw[1].w.r; sP[1][sp] = Pw[1].w.u; sP[2][sp] = Pw[1].w.v; sP[3][sp] = Pw[1].w.w;
+        sP[4][sp] = Pw[1].w.p; sP[5][sp] = Pw[1].w.e; sP[6][sp] = Pw[1].T;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) sG[i][sg] = Gw[0].g.g[i];
+        sG[12][sg] = Gw[0].g.mu; sG[13][sg] = Gw[0].g.k; sG[14][sg] = Gw[0].th;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) sX[x][sg] = Xw[0][x];
+      } else {
+        if (p_ok(J, K)) {
+          const long long c = pl + (long long)J * N[2] + K;
+          sP[0][sp] = P[c]; sP[1][sp] = P[cells + c]; sP[2][sp] = P[2 * cells + c];
+          sP[3][sp] = P[3 * cells + c]; sP[4][sp] = P[4 * cells + c];
+          sP[5][sp] = P[P_E * cells + c]; sP[6][sp] = P[P_T * cells + c];
+        }
+        if (g_ok(J, K)) {
+          const long long c = pl + (long long)J * N[2] + K;
+          for (int i = 0; i < NGW; ++i) sG[i][sg] = G[i * cells + c];
+          for (int x = 0; x < 3; ++x) sX[x][sg] = XC[x * cells + c];
+        }
+      }
+      // halo rings: 144 primitive slots (halo 2), 68 gradient/centroid slots (halo 1)
+      if (tid < 144) {
+        int sj, sk;
+        ring_slot(tid, 2, &sj, &sk);
+        const int jj = ng + j0 - 2 + sj, kk = ng + k0 - 2 + sk;
+        if (p_ok(jj, kk)) {
+          const long long c = pl + (long long)jj * N[2] + kk;
+          const int s = sj * HK + sk;
+          sP[0][s] = P[c]; sP[1][s] = P[cells + c]; sP[2][s] = P[2 * cells + c];
+          sP[3][s] = P[3 * cells + c]; sP[4][s] = P[4 * cells + c];
+          sP[5][s] = P[P_E * cells + c]; sP[6][s] = P[P_T * cells + c];
+        }
+      } else if (tid < 144 + 68) {
+        int sj, sk;
+        ring_slot(tid - 144, 1, &sj, &sk);
+        const int jj = ng + j0 - 1 + sj, kk = ng + k0 - 1 + sk;
+        if (g_ok(jj, kk)) {
+          const long long c = pl + (long long)jj * N[2] + kk;
+          const int s = sj * GK + sk;
+          for (int i = 0; i < NGW; ++i) sG[i][s] = G[i * cells + c];
+          for (int x = 0; x < 3; ++x) sX[x][s] = XC[x * cells + c];
+        }
+      }
+    }
+    __syncthreads();
+    // ---- face fluxes: 0 i face I+1 (registers), 1 j face J, 2 k face K, 3 tile-edge faces
+    const int ntask = first ? 1 : (tid < 64 ? 4 : 3);
+#pragma unroll 1
+    for (int task = 0; task < ntask; ++task) {
+      bool doit = act;
+      int d = 0, fI = I, fJ = J, fK = K;
+      CellWT<R> L2, L, Rc, R2;
+      CellGT<R> gL, gR;
+      R thL, thR, TL, TR;
+      double xL[3], xR[3];
+      if (task == 0) {
+        fI = I + 1;
+        L2 = Pw[0].w; L = Pw[1].w; Rc = Pw[2].w; R2 = Pw[3].w;
+        TL = Pw[1].T; TR = Pw[2].T;
+        gL = Gw[0].g; gR = Gw[1].g; thL = Gw[0].th; thR = Gw[1].th;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { xL[x] = Xw[0][x]; xR[x] = Xw[1][x]; }
+      } else {
+        int rj = tj, rk = tk;                        // tile coords of the face's right cell
+        if (task == 1) { d = 1; }
+        else if (task == 2) { d = 2; }
+        else if (tid < TK) { d = 1; rj = jl; rk = tid; doit = tid < kl; }
+        else if (tid < TK + TJ) { d = 2; rj = tid - TK; rk = kl; doit = (tid - TK) < jl; }
+        else doit = false;
+        fJ = ng + j0 + rj;
+        fK = ng + k0 + rk;
+        const int dj = d == 1, dk = d == 2;
+        const int pR = (rj + 2) * HK + (rk + 2), gR_ = (rj + 1) * GK + (rk + 1);
+        const int pst = dj * HK + dk, gst = dj * GK + dk;
+        if (doit) {
+          L2 = sp_w<R>(sP, pR - 2 * pst); L = sp_w<R>(sP, pR - pst); Rc = sp_w<R>(sP, pR);
+          R2 = sp_w<R>(sP, pR + pst);
+          TL = sP[6][pR - pst]; TR = sP[6][pR];
+          gL = sg_g<R>(sG, gR_ - gst); gR = sg_g<R>(sG, gR_);
+          thL = sG[14][gR_ - gst]; thR = sG[14][gR_];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) { xL[x] = sX[x][gR_ - gst]; xR[x] = sX[x][gR_]; }
+        }
+      }
+      double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (doit) {
+        const int cf = d == 0 ? fI : (d == 1 ? fJ : fK);
+        int side = -1;
+        if (cf == ng) side = 0;
+        else if (cf == ng + B.n[d]) side = 1;
+        if (side >= 0 && B.faces[2 * d + side].kind == WMX_BC_WALLMODEL) {
+          int t1, t2;
+          tangents(d, &t1, &t2);
+          const int cc[3] = {fI, fJ, fK};
+          const long long nw = (long long)B.n[t1] * B.n[t2];
+          const long long w = (long long)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
+          const double* W = B.wall[2 * d + side];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) F[v] = W[v * nw + w];
+        } else {
+          const long long fi = fidx(N, d, fI, fJ, fK), fc = fcount(N, d);
+          const double* S = B.S[d];
+          flux_core<R>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, S[fi], S[fc + fi],
+                       S[2 * fc + fi], F);
+        }
+      }
+      if (doit) {
+        if (task == 0) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Fhi[v] = F[v];
+        } else {
+          double* dst;
+          int o;
+          if (d == 1) { dst = &sFj[0][0]; o = (fJ - ng - j0) * TK + (fK - ng - k0); }
+          else { dst = &sFk[0][0]; o = (fJ - ng - j0) * (TK + 1) + (fK - ng - k0); }
+          const int stride = d == 1 ? (TJ + 1) * TK : TJ * (TK + 1);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) dst[v * stride + o] = F[v];
+        }
+      }
+    }
+    if (first) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Flo[v] = Fhi[v];
+      continue;                                      // uniform: no barrier skipped
+    }
+    __syncthreads();
+    if (act) {
+      const long long c = (long long)I * sI + (long long)J * N[2] + K;
+      const double vol = B.vol[c];
+      double r[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        double a = 0.0 - (Fhi[v] - Flo[v]);
+        a = a - (sFj[v][(tj + 1) * TK + tk] - sFj[v][tj * TK + tk]);
+        a = a - (sFk[v][tj * (TK + 1) + tk + 1] - sFk[v][tj * (TK + 1) + tk]);
+        r[v] = a / vol;
+      }
+      const double* qi = qin.p[t.x];
+      if (nd.n_sponge > 0) {
+        const double xcv = XC[c];
+        const int i = I - ng;
+        const long long mrow = (long long)i * n1 + j;
+        const long long mplane = (long long)n0 * n1;
+        for (int sp = 0; sp < nd.n_sponge; ++sp) {
+          const double* pr = nd.sponge[sp];
+          const double x_l = xcv / pr[4];
+          const double eta = fmin(fmax((x_l - pr[0]) / (pr[1] - pr[0]), 0.0), 1.0);
+          const double ramp = (eta * eta * eta) * (10.0 + eta * (-15.0 + 6.0 * eta));
+          const double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
+#pragma unroll
+          for (int v = 0; v < 5; ++v)
+            r[v] = r[v] + (-sigma) * (qi[v * cells + c] - B.sponge_mean[v * mplane + mrow]);
+        }
+      }
+      double* o = out.p[t.x];
+      if (MODE == 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = r[v];
+      } else if (MODE == 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = qi[v * cells + c] + dt * r[v];
+      } else {
+        const double* qz = q0.p[t.x];
+        const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
+        const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          o[v * cells + c] = a0 * qz[v * cells + c] + a1 * (qi[v * cells + c] + dt * r[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Flo[v] = Fhi[v];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------ flux
+__device__ __forceinline__ CellW load_w(const double* __restrict__ P, long long cells, long long c) {
+  CellW w;
+  w.r = P[c];
+  w.u = P[cells + c];
+  w.v = P[2 * cells + c];
+  w.w = P[3 * cells + c];
+  w.p = P[4 * cells + c];
+  w.e = P[7 * cells + c];
+  return w;
+}
+
+__device__ __forceinline__ CellG load_g(const double* __restrict__ G, long long cells, long long c) {
+  CellG g;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) g.g[i] = G[i * cells + c];
+  g.mu = G[G_MU * cells + c];
+  g.k = G[G_K * cells + c];
+  return g;
+}
+
+// Numerical flux (times area) through face f of direction d, f = coordinate of the
+// right cell (I, J, K); solver.py:88-93 with the wall override.
+__device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, int d, int I, int J,
+                                          int K, double* F) {
+  const int ng = B.ng;
+  const int cc[3] = {I, J, K};
+  const int f = cc[d];
+  int side = -1;
+  if (f == ng) side = 0;
+  else if (f == ng + B.n[d]) side = 1;
+  if (side >= 0 && B.faces[2 * d + side].kind == WMX_BC_WALLMODEL) {
+    int t1, t2;
+    tangents(d, &t1, &t2);
+    const long long nw = (long long)B.n[t1] * B.n[t2];
+    const long long w = (long long)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
+    const double* W = B.wall[2 * d + side];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = W[v * nw + w];
+    return;
+  }
+  const int* N = B.N;
+  const long long cells = B.cells;
+  const long long sd = d == 0 ? (long long)N[1] * N[2] : (d == 1 ? (long long)N[2] : 1LL);
+  const long long cR = cidx(N, I, J, K), cL = cR - sd;
+  const double* P = B.P;
+  const double* G = B.G;
+  const long long fi = fidx(N, d, I, J, K), fc = fcount(N, d);
+  const double* S = B.S[d];
+  const double Sx = S[fi], Sy = S[fc + fi], Sz = S[2 * fc + fi];
+  const CellW L = load_w(P, cells, cL), R = load_w(P, cells, cR);
+  const int sch = nd.scheme;
+  if (sch == WMX_SCHEME_KEP || sch == WMX_SCHEME_KEEP) {
+    CellW L2 = L, R2 = R;
+    if (nd.order == 4) { L2 = load_w(P, cells, cL - sd); R2 = load_w(P, cells, cR + sd); }
+    if (sch == WMX_SCHEME_KEP) central_flux<false>(nd.order, L2, L, R, R2, Sx, Sy, Sz, F);
+    else central_flux<true>(nd.order, L2, L, R, R2, Sx, Sy, Sz, F);
+  } else {
+    const CellW L2 = load_w(P, cells, cL - sd), R2 = load_w(P, cells, cR + sd);
+    if (sch == WMX_SCHEME_MUSCLHLLC) {
+      muscl_hllc_flux(nd.gas.gamma, nd.limiter, L2, L, R, R2, Sx, Sy, Sz, F);
+    } else {
+      // Hybrid: (1 - s) F_c + s F_u; a part whose weight is exactly 0 is not
+      // evaluated (bit-identical while the skipped flux is finite).
+      const double th = fmax(G[G_TH * cells + cL], G[G_TH * cells + cR]);
+      const double s = fmin(fmax((th - nd.th0) / (nd.th1 - nd.th0), 0.0), 1.0);
+      double Fc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Fu[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      if (s < 1.0) {
+        if (nd.central == WMX_SCHEME_KEEP) central_flux<true>(nd.order, L2, L, R, R2, Sx, Sy, Sz, Fc);
+        else central_flux<false>(nd.order, L2, L, R, R2, Sx, Sy, Sz, Fc);
+      }
+      if (s > 0.0) muscl_hllc_flux(nd.gas.gamma, nd.limiter, L2, L, R, R2, Sx, Sy, Sz, Fu);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) F[v] = (1.0 - s) * Fc[v] + s * Fu[v];
+    }
+  }
+  // viscous part (viscous.py:28-51)
+  const CellG gL = load_g(G, cells, cL), gR = load_g(G, cells, cR);
+  const double phL[4] = {L.u, L.v, L.w, P[P_T * cells + cL]};
+  const double phR[4] = {R.u, R.v, R.w, P[P_T * cells + cR]};
+  const double xL[3] = {B.xc[cL], B.xc[cells + cL], B.xc[2 * cells + cL]};
+  const double xR[3] = {B.xc[cR], B.xc[cells + cR], B.xc[2 * cells + cR]};
+  viscous_face(gL, gR, phL, phR, xL, xR, Sx, Sy, Sz, F);
+}
+
+// FP64 plain form: every face reads its stencil through the caches (no register
+// window), which keeps the kernel at 198 VGPRs / 2 waves per SIMD.
+// MODE 0: r = RHS; 1: q + dt r; 2: 3/4 q0 + 1/4 (q + dt r); 3: 1/3 q0 + 2/3 (q + dt r)
+template <int MODE>
+__global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks,
+                                              const int4* __restrict__ tiles, int ci_len,
+                                              NumDev nd, Ptrs qin, Ptrs q0, Ptrs out,
+                                              const double* __restrict__ d_dt, double dt_host) {
+  __shared__ double sFj[5][(TJ + 1) * TK];
+  __shared__ double sFk[5][TJ * (TK + 1)];
+  const int4 t = tiles[blockIdx.x];
+  const BlockDev& B = blks[t.x];
+  const int j0 = t.y, k0 = t.z, i0 = t.w;
+  const int i1 = min(i0 + ci_len, B.n[0]);
+  const int tid = threadIdx.x;
+  const int tj = tid / TK, tk = tid % TK;
+  const int j = j0 + tj, k = k0 + tk;
+  const bool act = j < B.n[1] && k < B.n[2];
+  const int jl = min(j0 + TJ, B.n[1]) - j0;
+  const int kl = min(k0 + TK, B.n[2]) - k0;
+  const int ng = B.ng;
+  const int J = ng + j, K = ng + k;
+  const double dt = d_dt ? d_dt[B.case_id] : dt_host;
+  const long long cells = B.cells;
+  double Flo[5], Fhi[5];
+  for (int i = i0; i < i1; ++i) {
+    const int I = ng + i;
+#pragma unroll 1
+    for (int task = (i == i0 ? 0 : 1); task < 5; ++task) {
+      if (task == 4 && tid >= 64) break;                 // tile-edge faces: wave 0 only
+      int d = 0, fI = I, fJ = J, fK = K;
+      bool doit = act;
+      if (task == 0) { d = 0; }
+      else if (task == 1) { d = 0; fI = I + 1; }
+      else if (task == 2) { d = 1; }
+      else if (task == 3) { d = 2; }
+      else {
+        if (tid < TK) { d = 1; fJ = ng + j0 + jl; fK = ng + k0 + tid; doit = tid < kl; }
+        else if (tid < TK + TJ) { d = 2; fJ = ng + j0 + (tid - TK); fK = ng + k0 + kl; doit = (tid - TK) < jl; }
+        else doit = false;
+      }
+      double F[5];
+      if (doit) face_flux(nd, B, d, fI, fJ, fK, F);
+      if (doit) {
+        if (task == 0) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Flo[v] = F[v];
+        } else if (task == 1) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Fhi[v] = F[v];
+        } else if (task == 2) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sFj[v][tj * TK + tk] = F[v];
+        } else if (task == 3) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sFk[v][tj * (TK + 1) + tk] = F[v];
+        } else if (tid < TK) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sFj[v][jl * TK + tid] = F[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sFk[v][(tid - TK) * (TK + 1) + kl] = F[v];
+        }
+      }
+    }
+    __syncthreads();
+    if (act) {
+      const long long c = cidx(B.N, I, J, K);
+      const double vol = B.vol[c];
+      double r[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        double a = 0.0 - (Fhi[v] - Flo[v]);
+        a = a - (sFj[v][(tj + 1) * TK + tk] - sFj[v][tj * TK + tk]);
+        a = a - (sFk[v][tj * (TK + 1) + tk + 1] - sFk[v][tj * (TK + 1) + tk]);
+        r[v] = a / vol;
+      }
+      const double* qi = qin.p[t.x];
+      if (nd.n_sponge > 0) {
+        // SmoothRampSponge, smooth_ramp.py:212-221
+        const double xcv = B.xc[c];
+        const long long mrow = (long long)i * B.n[1] + j;
+        const long long mplane = (long long)B.n[0] * B.n[1];
+        for (int sp = 0; sp < nd.n_sponge; ++sp) {
+          const double* pr = nd.sponge[sp];
+          double x_l = xcv / pr[4];
+          double eta = fmin(fmax((x_l - pr[0]) / (pr[1] - pr[0]), 0.0), 1.0);
+          double ramp = (eta * eta * eta) * (10.0 + eta * (-15.0 + 6.0 * eta));
+          double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
+#pragma unroll
+          for (int v = 0; v < 5; ++v)
+            r[v] = r[v] + (-sigma) * (qi[v * cells + c] - B.sponge_mean[v * mplane + mrow]);
+        }
+      }
+      double* o = out.p[t.x];
+      if (MODE == 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = r[v];
+      } else if (MODE == 1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = qi[v * cells + c] + dt * r[v];
+      } else {
+        const double* qz = q0.p[t.x];
+        const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
+        const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          o[v * cells + c] = a0 * qz[v * cells + c] + a1 * (qi[v * cells + c] + dt * r[v]);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Flo[v] = Fhi[v];
+    }
+    __syncthreads();
+  }
+}
+
+template __global__ void k_flux1<0>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
+                                   const double*, double);
+template __global__ void k_flux1<1>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
+                                   const double*, double);
+template __global__ void k_flux1<2>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
+                                   const double*, double);
+template __global__ void k_flux1<3>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
+                                   const double*, double);
+
+
+hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles, int ci_len,
+                       const NumDev& nd, Ptrs qin, Ptrs q0, Ptrs out, const double* d_dt,
+                       double dt, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  dim3 grid(ntiles), block(NT);
+  if (nd.prec == WMX_PREC_FP32_BULK) {
+    switch (mode) {
+      case 0: hipLaunchKernelGGL((k_flux2<float, 0>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+      case 1: hipLaunchKernelGGL((k_flux2<float, 1>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+      case 2: hipLaunchKernelGGL((k_flux2<float, 2>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+      default: hipLaunchKernelGGL((k_flux2<float, 3>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+    }
+    return hipGetLastError();
+  }
+  switch (mode) {
+    case 0: hipLaunchKernelGGL(k_flux1<0>, grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+    case 1: hipLaunchKernelGGL(k_flux1<1>, grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+    case 2: hipLaunchKernelGGL(k_flux1<2>, grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+    default: hipLaunchKernelGGL(k_flux1<3>, grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace wmx

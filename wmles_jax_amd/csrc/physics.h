@@ -100,230 +100,285 @@ WMX_HD void bc_farfield(const GasC& g, const double* src, const double* pinf, do
 }
 
 // --------------------------------------------------------------------- inviscid fluxes
-// Primitive record of one cell as the face kernels read it: rho, u, v, w, p, e_int.
-struct CellW {
-  double r, u, v, w, p, e;   // e = p / ((gamma-1) rho)
-};
+// The bulk face arithmetic is templated on R: R = double is the plain form; R = float
+// is the FP32-bulk form, in which p and e_int arrive gauge-shifted (p - p_ref,
+// e_int - e_ref) and the caller adds the shift's contribution back in FP64.
 
-// Kennedy-Gruber split flux incl. area, reference schemes.py:15-23.
-WMX_HD void kg_flux(const CellW& a, const CellW& b, double Sx, double Sy, double Sz, double* F) {
-  double rho = 0.5 * (a.r + b.r);
-  double u = 0.5 * (a.u + b.u), v = 0.5 * (a.v + b.v), w = 0.5 * (a.w + b.w);
-  double p = 0.5 * (a.p + b.p);
-  double ka = (a.u * a.u + a.v * a.v) + a.w * a.w;
-  double kb = (b.u * b.u + b.v * b.v) + b.w * b.w;
-  double ea = 0.5 * (((a.e + 0.5 * ka) + b.e) + 0.5 * kb);
-  double un = dot3(u, v, w, Sx, Sy, Sz);
-  double m = rho * un;
+// Primitive record of one cell as the face kernels read it: rho, u, v, w, p, e_int.
+template <class R>
+struct CellWT {
+  R r, u, v, w, p, e;   // e = p / ((gamma-1) rho)
+};
+using CellW = CellWT<double>;
+
+template <class R>
+WMX_HD R dot3r(R a0, R a1, R a2, R b0, R b1, R b2) {
+  return (a0 * b0 + a1 * b1) + a2 * b2;
+}
+
+// Kennedy-Gruber split flux incl. area, reference schemes.py:15-23.  *un = u_bar . S.
+template <class R>
+WMX_HD void kg_flux(const CellWT<R>& a, const CellWT<R>& b, R Sx, R Sy, R Sz, R* F, R* un_out) {
+  const R h = R(0.5);
+  R rho = h * (a.r + b.r);
+  R u = h * (a.u + b.u), v = h * (a.v + b.v), w = h * (a.w + b.w);
+  R p = h * (a.p + b.p);
+  R ka = (a.u * a.u + a.v * a.v) + a.w * a.w;
+  R kb = (b.u * b.u + b.v * b.v) + b.w * b.w;
+  R ea = h * (((a.e + h * ka) + b.e) + h * kb);
+  R un = dot3r(u, v, w, Sx, Sy, Sz);
+  R m = rho * un;
   F[0] = m;
   F[1] = m * u + p * Sx;
   F[2] = m * v + p * Sy;
   F[3] = m * w + p * Sz;
   F[4] = m * ea + p * un;
+  *un_out = un;
 }
 
 // KEEP two-point flux, reference schemes.py:25-55.
-WMX_HD void keep_flux(const CellW& a, const CellW& b, double Sx, double Sy, double Sz, double* F) {
-  double rho = 0.5 * (a.r + b.r);
-  double u = 0.5 * (a.u + b.u), v = 0.5 * (a.v + b.v), w = 0.5 * (a.w + b.w);
-  double p = 0.5 * (a.p + b.p);
-  double un = dot3(u, v, w, Sx, Sy, Sz);
-  double m = rho * un;
-  double K = m * (0.5 * dot3(a.u, a.v, a.w, b.u, b.v, b.w));
-  double I = (m * 0.5) * (a.e + b.e);
-  double unL = dot3(a.u, a.v, a.w, Sx, Sy, Sz);
-  double unR = dot3(b.u, b.v, b.w, Sx, Sy, Sz);
-  double P = 0.5 * (unR * a.p + unL * b.p);
+template <class R>
+WMX_HD void keep_flux(const CellWT<R>& a, const CellWT<R>& b, R Sx, R Sy, R Sz, R* F, R* un_out) {
+  const R h = R(0.5);
+  R rho = h * (a.r + b.r);
+  R u = h * (a.u + b.u), v = h * (a.v + b.v), w = h * (a.w + b.w);
+  R p = h * (a.p + b.p);
+  R un = dot3r(u, v, w, Sx, Sy, Sz);
+  R m = rho * un;
+  R K = m * (h * dot3r(a.u, a.v, a.w, b.u, b.v, b.w));
+  R I = (m * h) * (a.e + b.e);
+  R unL = dot3r(a.u, a.v, a.w, Sx, Sy, Sz);
+  R unR = dot3r(b.u, b.v, b.w, Sx, Sy, Sz);
+  R P = h * (unR * a.p + unL * b.p);
   F[0] = m;
   F[1] = m * u + p * Sx;
   F[2] = m * v + p * Sy;
   F[3] = m * w + p * Sz;
   F[4] = (K + I) + P;
+  *un_out = un;
 }
 
 // Central split form of order 2 or 4, reference schemes.py:57-70 (KEP), 72-98 (KEEP).
-template <bool KEEP>
-WMX_HD void central_flux(int order, const CellW& L2, const CellW& L, const CellW& R,
-                         const CellW& R2, double Sx, double Sy, double Sz, double* F) {
-  double F0[5];
-  if (KEEP) keep_flux(L, R, Sx, Sy, Sz, F0); else kg_flux(L, R, Sx, Sy, Sz, F0);
+// *un4 is the same order-2/4 combination of u_bar . S (for the gauge term).
+template <bool KEEP, class R>
+WMX_HD void central_flux(int order, const CellWT<R>& L2, const CellWT<R>& L, const CellWT<R>& Rc,
+                         const CellWT<R>& R2, R Sx, R Sy, R Sz, R* F, R* un4) {
+  R F0[5], u0;
+  if (KEEP) keep_flux(L, Rc, Sx, Sy, Sz, F0, &u0); else kg_flux(L, Rc, Sx, Sy, Sz, F0, &u0);
   if (order == 2) {
     for (int c = 0; c < 5; ++c) F[c] = F0[c];
+    *un4 = u0;
     return;
   }
-  double Fa[5], Fb[5];
+  R Fa[5], Fb[5], ua, ub;
   if (KEEP) {
-    keep_flux(L, R2, Sx, Sy, Sz, Fa);
-    keep_flux(L2, R, Sx, Sy, Sz, Fb);
+    keep_flux(L, R2, Sx, Sy, Sz, Fa, &ua);
+    keep_flux(L2, Rc, Sx, Sy, Sz, Fb, &ub);
   } else {
-    kg_flux(L, R2, Sx, Sy, Sz, Fa);
-    kg_flux(L2, R, Sx, Sy, Sz, Fb);
+    kg_flux(L, R2, Sx, Sy, Sz, Fa, &ua);
+    kg_flux(L2, Rc, Sx, Sy, Sz, Fb, &ub);
   }
-  for (int c = 0; c < 5; ++c)
-    F[c] = 2.0 * ((2.0 / 3.0) * F0[c] - (1.0 / 12.0) * (Fa[c] + Fb[c]));
+  const R c23 = R(2.0 / 3.0), c112 = R(1.0 / 12.0), two = R(2.0);
+  for (int c = 0; c < 5; ++c) F[c] = two * (c23 * F0[c] - c112 * (Fa[c] + Fb[c]));
+  *un4 = two * (c23 * u0 - c112 * (ua + ub));
 }
 
 // Limiters, reference schemes.py:100-105.
-WMX_HD double albada(double a, double b) {
-  const double eps = 1e-12;
-  return (a * (b * b + eps) + b * (a * a + eps)) / ((a * a + b * b) + 2.0 * eps);
+template <class R>
+WMX_HD R albada(R a, R b) {
+  const R eps = R(1e-12);
+  return (a * (b * b + eps) + b * (a * a + eps)) / ((a * a + b * b) + R(2.0) * eps);
 }
-WMX_HD double sgn(double a) { return a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0); }
-WMX_HD double minmod(double a, double b) {
-  return (0.5 * (sgn(a) + sgn(b))) * fmin(fabs(a), fabs(b));
+template <class R>
+WMX_HD R sgn(R a) { return a > R(0) ? R(1) : (a < R(0) ? R(-1) : R(0)); }
+template <class R>
+WMX_HD R minmod(R a, R b) {
+  return (R(0.5) * (sgn(a) + sgn(b))) * fmin(fabs(a), fabs(b));
 }
 
 // HLLC with Davis speeds and Toro's star state, times |S|; reference schemes.py:108-137.
-// wl, wr: [rho, u, v, w, p].
-WMX_HD void hllc_flux(double gamma, const double* wl, const double* wr, double Sx, double Sy,
-                      double Sz, double* F) {
-  double Sm = sqrt((Sx * Sx + Sy * Sy) + Sz * Sz);
-  double nx = Sx / Sm, ny = Sy / Sm, nz = Sz / Sm;
-  double rl = wl[0], ul = wl[1], vl = wl[2], zl = wl[3], pl = wl[4];
-  double rr = wr[0], ur = wr[1], vr = wr[2], zr = wr[3], pr = wr[4];
-  double unl = dot3(ul, vl, zl, nx, ny, nz), unr = dot3(ur, vr, zr, nx, ny, nz);
-  double cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
-  double sL = fmin(unl - cl, unr - cr), sR = fmax(unl + cl, unr + cr);
-  double ss = (((pr - pl) + (rl * unl) * (sL - unl)) - (rr * unr) * (sR - unr)) /
-              (rl * (sL - unl) - rr * (sR - unr));
-  double El = pl / (gamma - 1.0) + (0.5 * rl) * ((ul * ul + vl * vl) + zl * zl);
-  double Er = pr / (gamma - 1.0) + (0.5 * rr) * ((ur * ur + vr * vr) + zr * zr);
-  double r, u, v, z, p, un, E, s;
+// wl, wr: [rho, u, v, w, p] with the full pressure.
+template <class R>
+WMX_HD void hllc_flux(R gamma, const R* wl, const R* wr, R Sx, R Sy, R Sz, R* F) {
+  R Sm = sqrt((Sx * Sx + Sy * Sy) + Sz * Sz);
+  R nx = Sx / Sm, ny = Sy / Sm, nz = Sz / Sm;
+  R rl = wl[0], ul = wl[1], vl = wl[2], zl = wl[3], pl = wl[4];
+  R rr = wr[0], ur = wr[1], vr = wr[2], zr = wr[3], pr = wr[4];
+  R unl = dot3r(ul, vl, zl, nx, ny, nz), unr = dot3r(ur, vr, zr, nx, ny, nz);
+  R cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
+  R sL = fmin(unl - cl, unr - cr), sR = fmax(unl + cl, unr + cr);
+  R ss = (((pr - pl) + (rl * unl) * (sL - unl)) - (rr * unr) * (sR - unr)) /
+         (rl * (sL - unl) - rr * (sR - unr));
+  const R gm1 = gamma - R(1.0), h = R(0.5);
+  R El = pl / gm1 + (h * rl) * ((ul * ul + vl * vl) + zl * zl);
+  R Er = pr / gm1 + (h * rr) * ((ur * ur + vr * vr) + zr * zr);
+  R r, u, v, z, p, un, E, s;
   int branch;  // 0: FL, 1: F*L, 2: F*R, 3: FR
-  if (sL >= 0.0) branch = 0;
-  else if (ss >= 0.0) branch = 1;
-  else if (sR > 0.0) branch = 2;
+  if (sL >= R(0)) branch = 0;
+  else if (ss >= R(0)) branch = 1;
+  else if (sR > R(0)) branch = 2;
   else branch = 3;
   if (branch <= 1) { r = rl; u = ul; v = vl; z = zl; p = pl; un = unl; E = El; s = sL; }
   else { r = rr; u = ur; v = vr; z = zr; p = pr; un = unr; E = Er; s = sR; }
-  double m = r * un;
-  double Fp[5] = {m, m * u + p * nx, m * v + p * ny, m * z + p * nz, (E + p) * un};
+  R m = r * un;
+  R Fp[5] = {m, m * u + p * nx, m * v + p * ny, m * z + p * nz, (E + p) * un};
   if (branch == 1 || branch == 2) {
-    double f = r * (s - un) / (s - ss);
-    double e = E / r + (ss - un) * (ss + p / (r * (s - un)));
-    double d = ss - un;
-    double st[5] = {f, f * (u + d * nx), f * (v + d * ny), f * (z + d * nz), f * e};
-    double U[5] = {r, r * u, r * v, r * z, E};
+    R f = r * (s - un) / (s - ss);
+    R e = E / r + (ss - un) * (ss + p / (r * (s - un)));
+    R d = ss - un;
+    R st[5] = {f, f * (u + d * nx), f * (v + d * ny), f * (z + d * nz), f * e};
+    R U[5] = {r, r * u, r * v, r * z, E};
     for (int c = 0; c < 5; ++c) Fp[c] = Fp[c] + s * (st[c] - U[c]);
   }
   for (int c = 0; c < 5; ++c) F[c] = Fp[c] * Sm;
 }
 
 // MUSCL (primitive, index space) + positivity fallback + HLLC, reference schemes.py:140-152.
-WMX_HD void muscl_hllc_flux(double gamma, int limiter, const CellW& L2, const CellW& L,
-                            const CellW& R, const CellW& R2, double Sx, double Sy, double Sz,
-                            double* F) {
-  const double l2[5] = {L2.r, L2.u, L2.v, L2.w, L2.p};
-  const double l1[5] = {L.r, L.u, L.v, L.w, L.p};
-  const double r1[5] = {R.r, R.u, R.v, R.w, R.p};
-  const double r2[5] = {R2.r, R2.u, R2.v, R2.w, R2.p};
-  double wl[5], wr[5];
+// p_ref is added back to the (gauge-shifted) reconstructed pressures before HLLC.
+template <class R>
+WMX_HD void muscl_hllc_flux(R gamma, int limiter, const CellWT<R>& L2, const CellWT<R>& L,
+                            const CellWT<R>& Rc, const CellWT<R>& R2, R Sx, R Sy, R Sz, R p_ref,
+                            R* F) {
+  const R l2[5] = {L2.r, L2.u, L2.v, L2.w, L2.p};
+  const R l1[5] = {L.r, L.u, L.v, L.w, L.p};
+  const R r1[5] = {Rc.r, Rc.u, Rc.v, Rc.w, Rc.p};
+  const R r2[5] = {R2.r, R2.u, R2.v, R2.w, R2.p};
+  R wl[5], wr[5];
+  const R h = R(0.5);
   for (int c = 0; c < 5; ++c) {
-    double a = l1[c] - l2[c], b = r1[c] - l1[c], d = r2[c] - r1[c];
-    double la = limiter == 0 ? albada(a, b) : minmod(a, b);
-    double lb = limiter == 0 ? albada(b, d) : minmod(b, d);
-    wl[c] = l1[c] + 0.5 * la;
-    wr[c] = r1[c] - 0.5 * lb;
+    R a = l1[c] - l2[c], b = r1[c] - l1[c], d = r2[c] - r1[c];
+    R la = limiter == 0 ? albada(a, b) : minmod(a, b);
+    R lb = limiter == 0 ? albada(b, d) : minmod(b, d);
+    wl[c] = l1[c] + h * la;
+    wr[c] = r1[c] - h * lb;
   }
-  if (wl[0] <= 0.0 || wl[4] <= 0.0 || wr[0] <= 0.0 || wr[4] <= 0.0) {
+  wl[4] = wl[4] + p_ref;
+  wr[4] = wr[4] + p_ref;
+  if (wl[0] <= R(0) || wl[4] <= R(0) || wr[0] <= R(0) || wr[4] <= R(0)) {
     for (int c = 0; c < 5; ++c) { wl[c] = l1[c]; wr[c] = r1[c]; }
+    wl[4] = wl[4] + p_ref;
+    wr[4] = wr[4] + p_ref;
   }
   hllc_flux(gamma, wl, wr, Sx, Sy, Sz, F);
 }
 
 // ---------------------------------------------------------------------- viscous flux
 // Cell record the viscous flux reads: grad [u,v,w,T] (row-major g[v][c]), mu_eff, k_eff.
-struct CellG {
-  double g[12];
-  double mu, k;
+template <class R>
+struct CellGT {
+  R g[12];
+  R mu, k;
 };
+using CellG = CellGT<double>;
 
 // reference viscous.py:28-51 for one face; phi = [u, v, w, T]; adds into F.
-WMX_HD void viscous_face(const CellG& a, const CellG& b, const double* phiL, const double* phiR,
-                         const double* xcL, const double* xcR, double Sx, double Sy, double Sz,
-                         double* F) {
-  double dx0 = xcR[0] - xcL[0], dx1 = xcR[1] - xcL[1], dx2 = xcR[2] - xcL[2];
-  double dist = sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
-  double e0 = dx0 / dist, e1 = dx1 / dist, e2 = dx2 / dist;
-  double gf[12];
+// e (unit cell-to-cell vector) and dist come from FP64 centroids.
+template <class R>
+WMX_HD void viscous_face_e(const CellGT<R>& a, const CellGT<R>& b, const R* phiL, const R* phiR,
+                           R e0, R e1, R e2, R dist, R Sx, R Sy, R Sz, R* F) {
+  const R h = R(0.5);
+  R gf[12];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    double g0 = 0.5 * (a.g[3 * v + 0] + b.g[3 * v + 0]);
-    double g1 = 0.5 * (a.g[3 * v + 1] + b.g[3 * v + 1]);
-    double g2 = 0.5 * (a.g[3 * v + 2] + b.g[3 * v + 2]);
-    double corr = (phiR[v] - phiL[v]) / dist - ((g0 * e0 + g1 * e1) + g2 * e2);
+    R g0 = h * (a.g[3 * v + 0] + b.g[3 * v + 0]);
+    R g1 = h * (a.g[3 * v + 1] + b.g[3 * v + 1]);
+    R g2 = h * (a.g[3 * v + 2] + b.g[3 * v + 2]);
+    R corr = (phiR[v] - phiL[v]) / dist - ((g0 * e0 + g1 * e1) + g2 * e2);
     gf[3 * v + 0] = g0 + corr * e0;
     gf[3 * v + 1] = g1 + corr * e1;
     gf[3 * v + 2] = g2 + corr * e2;
   }
-  double muf = 0.5 * (a.mu + b.mu), kf = 0.5 * (a.k + b.k);
-  double div = (gf[0] + gf[4]) + gf[8];
-  double tdiv = (2.0 / 3.0) * div;
-  double t00 = muf * ((gf[0] + gf[0]) - tdiv), t11 = muf * ((gf[4] + gf[4]) - tdiv),
-         t22 = muf * ((gf[8] + gf[8]) - tdiv);
-  double t01 = muf * (gf[1] + gf[3]), t02 = muf * (gf[2] + gf[6]), t12 = muf * (gf[5] + gf[7]);
-  double ts0 = (t00 * Sx + t01 * Sy) + t02 * Sz;
-  double ts1 = (t01 * Sx + t11 * Sy) + t12 * Sz;
-  double ts2 = (t02 * Sx + t12 * Sy) + t22 * Sz;
-  double dTdn = (gf[9] * Sx + gf[10] * Sy) + gf[11] * Sz;
-  double uf0 = 0.5 * (phiL[0] + phiR[0]), uf1 = 0.5 * (phiL[1] + phiR[1]),
-         uf2 = 0.5 * (phiL[2] + phiR[2]);
-  F[0] = F[0] + 0.0;
+  R muf = h * (a.mu + b.mu), kf = h * (a.k + b.k);
+  R div = (gf[0] + gf[4]) + gf[8];
+  R tdiv = R(2.0 / 3.0) * div;
+  R t00 = muf * ((gf[0] + gf[0]) - tdiv), t11 = muf * ((gf[4] + gf[4]) - tdiv),
+    t22 = muf * ((gf[8] + gf[8]) - tdiv);
+  R t01 = muf * (gf[1] + gf[3]), t02 = muf * (gf[2] + gf[6]), t12 = muf * (gf[5] + gf[7]);
+  R ts0 = (t00 * Sx + t01 * Sy) + t02 * Sz;
+  R ts1 = (t01 * Sx + t11 * Sy) + t12 * Sz;
+  R ts2 = (t02 * Sx + t12 * Sy) + t22 * Sz;
+  R dTdn = (gf[9] * Sx + gf[10] * Sy) + gf[11] * Sz;
+  R uf0 = h * (phiL[0] + phiR[0]), uf1 = h * (phiL[1] + phiR[1]), uf2 = h * (phiL[2] + phiR[2]);
+  F[0] = F[0] + R(0.0);
   F[1] = F[1] + (-ts0);
   F[2] = F[2] + (-ts1);
   F[3] = F[3] + (-ts2);
   F[4] = F[4] + ((-((uf0 * ts0 + uf1 * ts1) + uf2 * ts2)) - kf * dTdn);
 }
 
+WMX_HD void viscous_face(const CellG& a, const CellG& b, const double* phiL, const double* phiR,
+                         const double* xcL, const double* xcR, double Sx, double Sy, double Sz,
+                         double* F) {
+  double dx0 = xcR[0] - xcL[0], dx1 = xcR[1] - xcL[1], dx2 = xcR[2] - xcL[2];
+  double dist = sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
+  viscous_face_e<double>(a, b, phiL, phiR, dx0 / dist, dx1 / dist, dx2 / dist, dist, Sx, Sy, Sz, F);
+}
+
+// Plain-form KEP/KEEP wrappers used by the FP64 kernels (no gauge).
+template <bool KEEP>
+WMX_HD void central_flux(int order, const CellW& L2, const CellW& L, const CellW& R, const CellW& R2,
+                         double Sx, double Sy, double Sz, double* F) {
+  double un4;
+  central_flux<KEEP, double>(order, L2, L, R, R2, Sx, Sy, Sz, F, &un4);
+}
+WMX_HD void muscl_hllc_flux(double gamma, int limiter, const CellW& L2, const CellW& L,
+                            const CellW& R, const CellW& R2, double Sx, double Sy, double Sz,
+                            double* F) {
+  muscl_hllc_flux<double>(gamma, limiter, L2, L, R, R2, Sx, Sy, Sz, 0.0, F);
+}
+
 // ------------------------------------------------------------------------------ SGS
 // g: velocity gradient g[i][j] = du_i/dx_j row-major (first 9 of CellG.g).
 // reference sgs.py:14-18
-WMX_HD double nut_smagorinsky(const double* g, double delta, double Cs) {
-  double ss = 0.0;
+template <class R>
+WMX_HD R nut_smagorinsky(const R* g, R delta, R Cs) {
+  R ss = R(0.0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double s = 0.5 * (g[3 * i + j] + g[3 * j + i]);
+      R s = R(0.5) * (g[3 * i + j] + g[3 * j + i]);
       ss = ss + s * s;
     }
-  double cd = Cs * delta;
-  return (cd * cd) * sqrt(2.0 * ss);
+  R cd = Cs * delta;
+  return (cd * cd) * sqrt(R(2.0) * ss);
 }
 
 // reference sgs.py:21-30
-WMX_HD double nut_wale(const double* g, double delta, double Cw) {
-  double ss = 0.0;
-  double g2[9];
+template <class R>
+WMX_HD R nut_wale(const R* g, R delta, R Cw) {
+  R ss = R(0.0);
+  R g2[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double s = 0.5 * (g[3 * i + j] + g[3 * j + i]);
+      R s = R(0.5) * (g[3 * i + j] + g[3 * j + i]);
       ss = ss + s * s;
       g2[3 * i + j] = (g[3 * i + 0] * g[0 + j] + g[3 * i + 1] * g[3 + j]) + g[3 * i + 2] * g[6 + j];
     }
-  double tr = (g2[0] + g2[4]) + g2[8];
-  double sdsd = 0.0;
+  R tr = (g2[0] + g2[4]) + g2[8];
+  R sdsd = R(0.0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double sd = 0.5 * (g2[3 * i + j] + g2[3 * j + i]) - (i == j ? tr / 3.0 : 0.0);
+      R sd = R(0.5) * (g2[3 * i + j] + g2[3 * j + i]) - (i == j ? tr / R(3.0) : R(0.0));
       sdsd = sdsd + sd * sd;
     }
-  double cd = Cw * delta;
-  double num = (cd * cd) * (sdsd * sqrt(sdsd));
-  double den = ((ss * ss) * sqrt(ss) + sqrt(sqrt(sdsd)) * sdsd) + 1e-30;
+  R cd = Cw * delta;
+  R num = (cd * cd) * (sdsd * sqrt(sdsd));
+  R den = ((ss * ss) * sqrt(ss) + sqrt(sqrt(sdsd)) * sdsd) + R(1e-30);
   return num / den;
 }
 
 // reference sgs.py:33-42
-WMX_HD double nut_vreman(const double* g, double delta, double Cs) {
-  double c = 2.5 * (Cs * Cs);
-  double d2 = delta * delta;
-  double b[9];
-  double aa = 0.0;
+template <class R>
+WMX_HD R nut_vreman(const R* g, R delta, R Cs) {
+  R c = R(2.5) * (Cs * Cs);
+  R d2 = delta * delta;
+  R b[9];
+  R aa = R(0.0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -332,16 +387,18 @@ WMX_HD double nut_vreman(const double* g, double delta, double Cs) {
                            g[3 * i + 2] * g[3 * j + 2]);
       aa = aa + g[3 * i + j] * g[3 * i + j];
     }
-  double B = ((((b[0] * b[4] - b[1] * b[1]) + b[0] * b[8]) - b[2] * b[2]) + b[4] * b[8]) - b[5] * b[5];
-  return c * sqrt(fmax(B, 0.0) / (aa + 1e-30));
+  R B = ((((b[0] * b[4] - b[1] * b[1]) + b[0] * b[8]) - b[2] * b[2]) + b[4] * b[8]) - b[5] * b[5];
+  return c * sqrt(fmax(B, R(0.0)) / (aa + R(1e-30)));
 }
 
 // reference viscous.py:22-25
-WMX_HD double ducros(const double* g) {
-  double div = (g[0] + g[4]) + g[8];
-  double w2 = (sq(g[7] - g[5]) + sq(g[2] - g[6])) + sq(g[3] - g[1]);
-  double d2 = div * div;
-  return d2 / ((d2 + w2) + 1e-30);
+template <class R>
+WMX_HD R ducros(const R* g) {
+  R div = (g[0] + g[4]) + g[8];
+  R a = g[7] - g[5], b = g[2] - g[6], c = g[3] - g[1];
+  R w2 = (a * a + b * b) + c * c;
+  R d2 = div * div;
+  return d2 / ((d2 + w2) + R(1e-30));
 }
 
 // ------------------------------------------------------------------------ wall model

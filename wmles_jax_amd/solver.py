@@ -102,7 +102,15 @@ class Solver:
     """solver.py:39-232."""
 
     def __init__(self, mesh, gas, scheme, sgs="vreman", cfl=0.8, sgs_kwargs=None,
-                 constraints=(), sources=(), bc_passes=2):
+                 constraints=(), sources=(), bc_passes=2, precision="fp64", gauge=None):
+        """precision: "fp64" (plain form) or "fp32" (FP32-bulk: FP32 flux, gradient and
+        SGS arithmetic; FP64 state, metrics, reductions, stage updates and wall model).
+        gauge: (p_ref, T_ref) of the FP32-bulk records; default: the interior means of
+        the first state the solver steps."""
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self._gauge = None
         self.mesh, self.blocks, self.gas = mesh, mesh.blocks, gas
         self.geoms = mesh.geoms
         if self.geoms is None:
@@ -122,7 +130,8 @@ class Solver:
         self._keep = []
         descs = [block_desc(b, mesh.index, self._ctx, g) for b, g in zip(self.blocks, self.geoms)]
         self._descs = (_lib.BlockDesc * len(descs))(*descs)
-        self._nd = numerics_desc(gas, scheme, self.sgs, cfl, bc_passes, native, self.sources)
+        self._nd = numerics_desc(gas, scheme, self.sgs, cfl, bc_passes, native, self.sources,
+                                 precision=_lib.PREC_FP32_BULK if precision == "fp32" else _lib.PREC_FP64)
         inflows = (_lib.InflowDesc * max(1, len(self._ctx.inflows)))()
         for i, inf in enumerate(self._ctx.inflows):
             inflows[i] = inf.desc(self._dev)
@@ -141,6 +150,20 @@ class Solver:
         _lib.check(_lib.lib.wmx_plan_wall_faces(plan, C.byref(cnt), rec))
         self._wall_faces = [tuple(rec[5 * i:5 * i + 5]) for i in range(cnt.value)]
         self._dt_buf = torch.empty(16, dtype=torch.float64, device=self._dev)
+        if gauge is not None:
+            self.set_gauge(*gauge)
+
+    def set_gauge(self, p_ref, T_ref):
+        """Reference pressure / temperature subtracted in the FP32-bulk records."""
+        self._gauge = (float(p_ref), float(T_ref))
+        _lib.check(_lib.lib.wmx_plan_set_gauge(self._plan, self._gauge[0], self._gauge[1]))
+
+    def _ensure_gauge(self, Q):
+        if self.precision != "fp32" or self._gauge is not None:
+            return
+        blk, q = self.blocks[0], Q[0]
+        w, T = self.gas.cons_to_prim(blk.interior(q))
+        self.set_gauge(float(w[4].mean()), float(T.mean()))
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -180,9 +203,11 @@ class Solver:
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES) if cnt[i]}
 
     def fill_phases(self):
-        n = C.c_int32()
+        """(kernel phases, sweep passes) of the ghost sweep after plan analysis."""
+        n, k = C.c_int32(), C.c_int32()
         _lib.check(_lib.lib.wmx_plan_fill_phases(self._plan, C.byref(n)))
-        return n.value
+        _lib.check(_lib.lib.wmx_plan_fill_passes(self._plan, C.byref(k)))
+        return n.value, k.value
 
     # ------------------------------------------------------------------ init
     def init_state(self, prim_fn):
@@ -205,6 +230,7 @@ class Solver:
     def rhs(self, Q, time=0.0):
         """RHS with zero ghosts (solver.py:97-106,159-160)."""
         work = tuple(q.clone() for q in self._as_state(Q))
+        self._ensure_gauge(work)
         R = tuple(torch.empty_like(q) for q in work)
         _lib.check(_lib.lib.wmx_rhs(self._plan, _lib.ptr_array(work), _lib.ptr_array(R), float(time),
                                     _lib.stream_ptr()))
@@ -221,6 +247,7 @@ class Solver:
     def advance(self, Q, dt, time=0.0):
         """One SSP-RK3 step + constraints (solver.py:123-133,156-157)."""
         Q = self._as_state(Q)
+        self._ensure_gauge(Q)
         out = tuple(torch.empty_like(q) for q in Q)
         if isinstance(dt, torch.Tensor):
             d = dt.reshape(-1).to(device=self._dev, dtype=torch.float64)
