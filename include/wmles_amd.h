@@ -183,6 +183,16 @@ int wmx_rhs(wmx_plan* plan, double* const* q, double* const* r, double time, voi
    Replaces Solver.advance / _advance_staged (wmles/solver.py:109-133,156-157). */
 int wmx_advance(wmx_plan* plan, double* const* q0, double* const* qout, double dt,
                 const double* d_dt, double time, void* stream);
+/* The reference driver's step (examples/channel_wmles.py:124-125: max_dt then advance)
+   with dt kept on the device: d_dt[case] <- CFL dt of q0, then qout = advance(q0).
+   use_graph != 0 (and a non-default stream, and no time-dependent inflow): the launch
+   sequence is captured once per (q0, qout, d_dt, time) into a hipGraph (at most 4
+   kept per plan) and replayed on later calls with the same pointers. */
+int wmx_step(wmx_plan* plan, double* const* q0, double* const* qout, double* d_dt, double time,
+             int32_t use_graph, void* stream);
+/* Flux-kernel tiling (autotuner knob): ci_len i-planes per marching tile; order 0
+   (block/i/j/k) or 1 (contiguous spans interleaved across the 8 XCDs). */
+int wmx_plan_set_tiling(wmx_plan* plan, int32_t ci_len, int32_t order);
 /* CFL time step (per case when the plan is an ensemble) into d_dt[n_cases].
    Replaces Solver.max_dt / _dt (wmles/solver.py:135-154). */
 int wmx_max_dt(wmx_plan* plan, double* const* q, double* d_dt, void* stream);

@@ -1,0 +1,76 @@
+"""Batching form on the device: graph-captured steps and merged-case ensembles must
+reproduce the per-case, launch-by-launch step bit for bit."""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a HIP device", allow_module_level=True)
+
+import wmles_jax_amd as wmx  # noqa: E402
+
+
+def _channel(nblocks, nx, ny, nz, Re_b, seed, splits=None, precision="fp64"):
+    spec, arrays = cases.channel_spec(nblocks, nx, ny, nz, {"kind": "KEP", "order": 2}, "wale",
+                                      Re_b=Re_b, splits=splits)
+    mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": precision})
+    pr = spec["props"]
+    return s, (lambda xc, pr=pr, seed=seed: cases.channel_prim(xc, pr, seed))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_step_equals_max_dt_plus_advance(precision):
+    s, fn = _channel(2, 32, 16, 16, 2e4, 0, precision=precision)
+    Q0 = s.init_state(fn)
+    A, B = Q0, Q0
+    for _ in range(4):                       # graph captured on the 1st/2nd call, replayed after
+        A, dta = s.step(A, graph=True)
+        dtb = s.max_dt(B)
+        B = s.advance(B, dtb)
+    torch.cuda.synchronize()
+    assert float(dta) == float(dtb)
+    for a, b in zip(A, B):
+        assert torch.equal(a, b)
+
+
+def test_ensemble_equals_individual_cases():
+    specs = [(2, 32, 16, 16, 2e4, 0, None), (3, 48, 16, 8, 8e3, 1, [0, 10, 30, 48]),
+             (2, 24, 24, 16, 3e4, 2, [0, 9, 24])]
+    solvers, fns = [], []
+    for nb, nx, ny, nz, re, seed, sp in specs:
+        s, fn = _channel(nb, nx, ny, nz, re, seed, sp)
+        solvers.append(s)
+        fns.append(fn)
+    ens = wmx.EnsembleSolver(solvers)
+    Qs = ens.init_state(fns)
+    ref = [Q for Q in Qs]
+    for _ in range(3):
+        Qs, dts = ens.step(Qs, graph=True)
+        new = []
+        for c, (s, Q) in enumerate(zip(solvers, ref)):
+            dt = s.max_dt(Q)
+            assert float(dt) == float(dts[c])
+            new.append(s.advance(Q, dt))
+        ref = new
+    torch.cuda.synchronize()
+    for Qe, Qr in zip(Qs, ref):
+        for a, b in zip(Qe, Qr):
+            assert torch.equal(a, b)
+
+
+def test_autotuner_picks_a_candidate(tmp_path, monkeypatch):
+    monkeypatch.setenv("WMX_TUNE_CACHE", str(tmp_path / "t.json"))
+    s, fn = _channel(1, 48, 32, 32, 2e4, 0)
+    Q = s.init_state(fn)
+    best, timings = wmx.autotune.tune(s, Q, steps=1, candidates=((8, 0), (16, 1)))
+    assert best in ((8, 0), (16, 1)) and len(timings) == 2
+    best2, _ = wmx.autotune.tune(s, Q)          # cached
+    assert best2 == best
+    A, _ = s.step(Q, graph=False)
+    s.set_tiling(16, 0)
+    B, _ = s.step(Q, graph=False)
+    for a, b in zip(A, B):                       # tiling changes scheduling, not results
+        assert torch.equal(a, b)

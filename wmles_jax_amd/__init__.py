@@ -17,9 +17,11 @@ from .sgs import make_sgs
 from .solver import Solver, ConstantMassFlux
 from . import smooth_ramp
 from .smooth_ramp import SmoothRampInflow, SmoothRampSponge, flow_properties
+from .ensemble import EnsembleSolver
+from . import autotune
 
 __all__ = ["HAS_JAX", "Gas", "Block", "Mesh", "Geom", "Interface", "FACE_NAMES", "sl", "BC",
            "SlipWall", "NoSlipWall", "Dirichlet", "CharacteristicFarfield", "Extrapolate",
            "WallModel", "EquilibriumWallModel", "KEP", "KEEP", "MUSCLHLLC", "Hybrid", "make_sgs",
            "Solver", "ConstantMassFlux", "smooth_ramp", "SmoothRampInflow", "SmoothRampSponge",
-           "flow_properties"]
+           "flow_properties", "EnsembleSolver", "autotune"]

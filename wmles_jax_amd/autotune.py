@@ -1,0 +1,73 @@
+"""Flux-kernel tiling autotuner (project-written).
+
+Times a few steps of a solver (or ensemble) for each candidate tiling -- i-planes per
+marching tile x XCD-interleaved tile order -- with HIP events around the flux kernel,
+keeps the fastest, and caches the choice per (device, form, block sizes) in a JSON file
+(default ~/.cache/wmles_jax_amd/tiling.json, override with WMX_TUNE_CACHE).
+"""
+import json
+import os
+
+import torch
+
+CANDIDATES = ((8, 0), (16, 0), (32, 0), (8, 1), (16, 1), (32, 1))
+
+
+def _cache_path():
+    return os.environ.get("WMX_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "wmles_jax_amd", "tiling.json"))
+
+
+def _key(obj):
+    solvers = getattr(obj, "solvers", [obj])
+    dims = [tuple(b.n) for s in solvers for b in s.blocks]
+    scheme = type(solvers[0].scheme).__name__
+    return f"{torch.cuda.get_device_name()}|{solvers[0].precision}|{scheme}|{dims}"
+
+
+def tune(obj, Q, steps=3, candidates=CANDIDATES, use_cache=True):
+    """Pick the fastest tiling for `obj` (Solver or EnsembleSolver) stepping state Q.
+    Returns ((ci_len, order), {candidate: ms per flux launch})."""
+    path = _cache_path()
+    key = _key(obj)
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            cache = json.load(open(path))
+        except ValueError:
+            cache = {}
+    if use_cache and key in cache:
+        ci, order = cache[key]["best"]
+        obj.set_tiling(ci, order)
+        return (ci, order), cache[key]["timings"]
+    plan_owner = getattr(obj, "solvers", None)
+    timings = {}
+    for ci, order in candidates:
+        obj.set_tiling(ci, order)
+        Qw = Q
+        Qw, _ = obj.step(Qw, graph=False)            # warm-up
+        torch.cuda.synchronize()
+        prof = obj if plan_owner is None else None
+        if prof is not None:
+            prof.profile(("flux",))
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(steps):
+            Qw, _ = obj.step(Qw, graph=False)
+        t1.record()
+        torch.cuda.synchronize()
+        if prof is not None:
+            ms, n = prof.profile_read().get("flux", (0.0, 0))
+            prof.profile(())
+            timings[f"{ci},{order}"] = ms / max(n, 1)
+        else:
+            timings[f"{ci},{order}"] = t0.elapsed_time(t1) / steps
+    best = min(timings, key=timings.get)
+    ci, order = (int(v) for v in best.split(","))
+    obj.set_tiling(ci, order)
+    if use_cache:
+        cache[key] = {"best": [ci, order], "timings": timings}
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        json.dump(cache, open(path, "w"), indent=1)
+    return (ci, order), timings

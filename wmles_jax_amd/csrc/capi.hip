@@ -24,7 +24,6 @@ int fail(const std::string& m) {
     if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr int CI_LEN = 16;          // i-planes per marching tile
 constexpr size_t ALIGN = 256;
 
 size_t align_up(size_t x) { return (x + ALIGN - 1) / ALIGN * ALIGN; }
@@ -81,6 +80,17 @@ struct wmx_plan {
   // flux tiles
   std::vector<int4> tiles;
   int4* d_tiles = nullptr;
+  int ci_len = 16;                    // i-planes per marching tile
+  int tile_order = 0;                 // 0: block/i/j/k order, 1: XCD-interleaved
+  // captured steps: (q0, qout, d_dt, time) -> graph of max_dt + advance
+  struct GraphEntry {
+    std::vector<double*> q0, qout;
+    const double* d_dt;
+    double time;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  bool capturing = false;
   // per-kernel workgroup prefix tables
   std::vector<int> pre_all, pre_box, pre_int, pre_shell;
   int *d_pre_all = nullptr, *d_pre_box = nullptr, *d_pre_int = nullptr, *d_pre_shell = nullptr;
@@ -123,7 +133,7 @@ hipEvent_t next_event(wmx_plan* p) {
 // bracket one launch of class `cls` with events when profiling is on
 template <class F>
 hipError_t timed(wmx_plan* p, int cls, hipStream_t s, F&& launch) {
-  if (!(p->prof_mask & (1 << cls))) return launch();
+  if (!(p->prof_mask & (1 << cls)) || p->capturing) return launch();
   const size_t i0 = p->ev_used;
   hipEvent_t a = next_event(p), b = next_event(p);
   if (!a || !b) return launch();
@@ -296,6 +306,31 @@ int build_phases(wmx_plan& p, int passes) {
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Flux tiles: per block (i-chunk, j-tile, k-tile).  Order 1 interleaves contiguous
+// spans of the tile list across the 8 XCDs (workgroup w runs on XCD w % 8), so the
+// tiles one XCD runs are neighbours and share its L2.
+void build_tiles(wmx_plan* p) {
+  std::vector<int4> t;
+  for (int b = 0; b < p->nb; ++b) {
+    const BlockDev& B = p->hb[b];
+    for (int i0 = 0; i0 < B.n[0]; i0 += p->ci_len)
+      for (int j0 = 0; j0 < B.n[1]; j0 += 16)
+        for (int k0 = 0; k0 < B.n[2]; k0 += 16) t.push_back(make_int4(b, j0, k0, i0));
+  }
+  if (p->tile_order == 1) {
+    const size_t n = t.size(), nx = 8, span = (n + nx - 1) / nx;
+    std::vector<int4> o;
+    o.reserve(n);
+    for (size_t r = 0; r < span; ++r)
+      for (size_t x = 0; x < nx; ++x) {
+        const size_t src = x * span + r;
+        if (src < n) o.push_back(t[src]);
+      }
+    t.swap(o);
+  }
+  p->tiles.swap(t);
+}
+
 NumDev make_numdev(const wmx_numerics_desc* nm) {
   NumDev nd;
   std::memset(&nd, 0, sizeof(nd));
@@ -459,13 +494,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
         p->wall.push_back(WallOp{b, f / 2, f % 2, cnt});
         p->wall_max = std::max(p->wall_max, cnt);
       }
-  // flux tiles: per block (i-chunk, j-tile, k-tile)
-  for (int b = 0; b < nblocks; ++b) {
-    const BlockDev& B = p->hb[b];
-    for (int i0 = 0; i0 < B.n[0]; i0 += CI_LEN)
-      for (int j0 = 0; j0 < B.n[1]; j0 += 16)
-        for (int k0 = 0; k0 < B.n[2]; k0 += 16) p->tiles.push_back(make_int4(b, j0, k0, i0));
-  }
+  build_tiles(p);
   // prefix tables
   std::vector<long long> c_all, c_box, c_int, c_shell;
   for (auto& B : p->hb) {
@@ -534,6 +563,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
 int wmx_plan_destroy(wmx_plan* p) {
   if (!p) return 0;
   for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
+  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.exec);
   (void)hipFree(p->d_blks);
   (void)hipFree(p->d_ops);
   (void)hipFree(p->d_wall);
@@ -702,7 +732,7 @@ int wmx_rhs(wmx_plan* p, double* const* q, double* const* r, double time, void* 
   hipStream_t s = as_stream(stream);
   Ptrs pq = make_ptrs(p, q), pr = make_ptrs(p, r);
   if (run_cells(p, pq, time, nullptr, 0.0, s)) return 1;
-  WMX_CHECK(launch_flux(0, p->d_blks, p->d_tiles, (int)p->tiles.size(), CI_LEN, p->nd, pq, pq, pr,
+  WMX_CHECK(launch_flux(0, p->d_blks, p->d_tiles, (int)p->tiles.size(), p->ci_len, p->nd, pq, pq, pr,
                         nullptr, 0.0, s));
   WMX_CHECK(launch_shell_finish(0, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), pq, pr,
                                 nullptr, 0.0, s));
@@ -740,11 +770,11 @@ int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, 
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
   if (run_cells(p, P0, time, d_dt, 0.0, s)) return 1;
-  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, P0, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
-  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t2, d_dt, 0.5, s)) return 1;
-  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(3, p->d_blks, p->d_tiles, ntiles, CI_LEN, p->nd, PO, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(3, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
   WMX_CHECK(timed(p, K_SHELL, s, [&] {
     return launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s);
   }));
@@ -763,6 +793,68 @@ int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
       e = launch_case_final(0, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part, nullptr, d_dt, s);
     return e;
   }));
+  return 0;
+}
+
+int wmx_plan_set_tiling(wmx_plan* p, int32_t ci_len, int32_t order) {
+  if (!p) return fail("null plan");
+  if (ci_len < 1 || ci_len > 1024) return fail("ci_len must be in [1, 1024]");
+  if (order != 0 && order != 1) return fail("order must be 0 or 1");
+  p->ci_len = ci_len;
+  p->tile_order = order;
+  build_tiles(p);
+  (void)hipFree(p->d_tiles);
+  p->d_tiles = nullptr;
+  if (upload(p->tiles, &p->d_tiles)) return 1;
+  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.exec);
+  p->graphs.clear();
+  return 0;
+}
+
+int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, double time,
+             int32_t use_graph, void* stream) {
+  if (check_plan(p)) return 1;
+  if (!d_dt) return fail("null d_dt");
+  hipStream_t s = as_stream(stream);
+  bool has_inflow = false;
+  for (auto& ph : p->phase_inflow) has_inflow |= !ph.empty();
+  if (!use_graph || has_inflow || s == nullptr) {
+    if (wmx_max_dt(p, q0, d_dt, stream)) return 1;
+    return wmx_advance(p, q0, qout, 0.0, d_dt, time, stream);
+  }
+  for (auto& g : p->graphs) {
+    bool same = g.d_dt == d_dt && g.time == time;
+    for (int b = 0; same && b < p->nb; ++b) same = g.q0[b] == q0[b] && g.qout[b] == qout[b];
+    if (same) {
+      WMX_CHECK(hipGraphLaunch(g.exec, s));
+      return 0;
+    }
+  }
+  hipGraph_t graph;
+  WMX_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  p->capturing = true;
+  int rc = wmx_max_dt(p, q0, d_dt, stream);
+  if (!rc) rc = wmx_advance(p, q0, qout, 0.0, d_dt, time, stream);
+  p->capturing = false;
+  hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc) return 1;
+  if (e != hipSuccess) return fail(std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(std::string("graph instantiate: ") + hipGetErrorString(e));
+  if (p->graphs.size() >= 4) {
+    (void)hipGraphExecDestroy(p->graphs.front().exec);
+    p->graphs.erase(p->graphs.begin());
+  }
+  wmx_plan::GraphEntry ge;
+  ge.q0.assign(q0, q0 + p->nb);
+  ge.qout.assign(qout, qout + p->nb);
+  ge.d_dt = d_dt;
+  ge.time = time;
+  ge.exec = exec;
+  p->graphs.push_back(ge);
+  WMX_CHECK(hipGraphLaunch(exec, s));
   return 0;
 }
 

@@ -1,0 +1,152 @@
+"""Ensembles of independent cases advanced together (batching form).
+
+`EnsembleSolver(solvers)` merges the blocks of several cases (each set up as an
+ordinary `Solver`: its own mesh, BCs, mass-flux target) into one HIP plan.  Every
+kernel of the step then covers all blocks of all cases in a single launch (one launch
+per kernel class and RK stage for the whole ensemble, instead of one per case), with
+per-case CFL time steps and per-case mass-flux reductions, and the step can be
+replayed as one captured hipGraph.  Numerics (gas, scheme, SGS, CFL, precision) must be
+common to the cases; geometry, sizes, block splits, BCs and targets may differ.
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .solver import numerics_desc
+
+
+class EnsembleSolver:
+    def __init__(self, solvers):
+        solvers = list(solvers)
+        if not 1 <= len(solvers) <= 16:
+            raise ValueError("an ensemble holds 1..16 cases")
+        s0 = solvers[0]
+        for s in solvers[1:]:
+            same = (s.gas == s0.gas and type(s.scheme) is type(s0.scheme) and s.sgs.code == s0.sgs.code
+                    and s.sgs.coef == s0.sgs.coef and s.cfl == s0.cfl and s.precision == s0.precision
+                    and s.bc_passes == s0.bc_passes and not s.sources and not s0.sources)
+            if not same:
+                raise ValueError("ensemble cases must share gas, scheme, SGS, CFL, precision and have no sources")
+        nblk = sum(len(s.blocks) for s in solvers)
+        if nblk > _lib.WMX_MAX_BLOCKS:
+            raise ValueError(f"an ensemble plan holds at most {_lib.WMX_MAX_BLOCKS} blocks")
+        self.solvers = solvers
+        self.precision = s0.precision
+        self._dev = s0._dev
+        descs = (_lib.BlockDesc * nblk)()
+        self._offsets, k = [], 0
+        for c, s in enumerate(solvers):
+            self._offsets.append(k)
+            for d in s._descs:
+                nd = _lib.BlockDesc()
+                C.memmove(C.byref(nd), C.byref(d), C.sizeof(_lib.BlockDesc))
+                nd.case_id = c
+                for f in range(6):
+                    if nd.faces[f].kind == _lib.FACE_INTERFACE:
+                        nd.faces[f].nb += k
+                    if nd.faces[f].kind == _lib.BC_INFLOW:
+                        raise ValueError("time-dependent inflow cases are not batched")
+                descs[k] = nd
+                k += 1
+        self._descs = descs
+        cons = []
+        for s in solvers:
+            if len(s._native_constraints) != 1 or s._py_constraints:
+                raise ValueError("each ensemble case needs exactly one ConstantMassFlux constraint")
+            cons.append(s._native_constraints[0])
+        self._nd = numerics_desc(s0.gas, s0.scheme, s0.sgs, s0.cfl, s0.bc_passes, cons, (),
+                                 n_cases=len(solvers),
+                                 precision=_lib.PREC_FP32_BULK if self.precision == "fp32" else _lib.PREC_FP64)
+        plan = C.c_void_p()
+        _lib.check(_lib.lib.wmx_plan_create(nblk, descs, C.byref(self._nd), 0, None, C.byref(plan)))
+        self._plan = plan
+        nbytes = C.c_size_t()
+        _lib.check(_lib.lib.wmx_plan_workspace_bytes(plan, C.byref(nbytes)))
+        self._ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=self._dev)
+        _lib.check(_lib.lib.wmx_plan_bind_workspace(plan, C.c_void_p(self._ws.data_ptr()), nbytes.value))
+        self._stream = torch.cuda.Stream(device=self._dev)
+        self._dt_dev = torch.empty(16, dtype=torch.float64, device=self._dev)
+        self._gauge = None
+
+    def __del__(self):
+        plan = getattr(self, "_plan", None)
+        if plan:
+            try:
+                torch.cuda.synchronize()
+                _lib.lib.wmx_plan_destroy(plan)
+            except Exception:
+                pass
+
+    @property
+    def n_cases(self):
+        return len(self.solvers)
+
+    def cells(self):
+        return sum(int(b.n[0] * b.n[1] * b.n[2]) for s in self.solvers for b in s.blocks)
+
+    def _flat(self, Qs):
+        if len(Qs) != self.n_cases:
+            raise ValueError("one state tuple per case")
+        flat = []
+        for s, Q in zip(self.solvers, Qs):
+            flat.extend(s._as_state(Q))
+        return flat
+
+    def _split(self, flat):
+        out, k = [], 0
+        for s in self.solvers:
+            out.append(tuple(flat[k:k + len(s.blocks)]))
+            k += len(s.blocks)
+        return out
+
+    def _ensure_gauge(self, Qs):
+        if self.precision != "fp32" or self._gauge is not None:
+            return
+        s0 = self.solvers[0]
+        s0._ensure_gauge(s0._as_state(Qs[0]))
+        self._gauge = s0._gauge
+        _lib.check(_lib.lib.wmx_plan_set_gauge(self._plan, *self._gauge))
+
+    def init_state(self, prim_fns):
+        return [s.init_state(f) for s, f in zip(self.solvers, prim_fns)]
+
+    def max_dt(self, Qs):
+        """Per-case CFL time steps, device tensor (n_cases,)."""
+        flat = self._flat(Qs)
+        out = torch.empty(16, dtype=torch.float64, device=self._dev)
+        _lib.check(_lib.lib.wmx_max_dt(self._plan, _lib.ptr_array(flat), C.c_void_p(out.data_ptr()),
+                                       _lib.stream_ptr()))
+        return out[:self.n_cases]
+
+    def advance(self, Qs, dts, time=0.0):
+        """All cases one SSP-RK3 step with their own dt (device tensor (n_cases,))."""
+        flat = self._flat(Qs)
+        self._ensure_gauge(Qs)
+        d = torch.as_tensor(dts, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
+        if d.numel() != self.n_cases:
+            raise ValueError("one dt per case")
+        out = [torch.empty_like(q) for q in flat]
+        _lib.check(_lib.lib.wmx_advance(self._plan, _lib.ptr_array(flat), _lib.ptr_array(out), 0.0,
+                                        C.c_void_p(d.data_ptr()), float(time), _lib.stream_ptr()))
+        return self._split(out)
+
+    def step(self, Qs, graph=True):
+        """max_dt + advance for every case; graph-captured replay when pointers repeat."""
+        flat = self._flat(Qs)
+        self._ensure_gauge(Qs)
+        cur = torch.cuda.current_stream(self._dev)
+        self._stream.wait_stream(cur)
+        with torch.cuda.stream(self._stream):
+            out = [torch.empty_like(q) for q in flat]
+            _lib.check(_lib.lib.wmx_step(self._plan, _lib.ptr_array(flat), _lib.ptr_array(out),
+                                         C.c_void_p(self._dt_dev.data_ptr()), 0.0, int(bool(graph)),
+                                         _lib.stream_ptr(self._stream)))
+            dts = self._dt_dev[:self.n_cases].clone()
+        cur.wait_stream(self._stream)
+        for q in flat:
+            q.record_stream(self._stream)
+        return self._split(out), dts
+
+    def set_tiling(self, ci_len=16, order=0):
+        _lib.check(_lib.lib.wmx_plan_set_tiling(self._plan, int(ci_len), int(order)))

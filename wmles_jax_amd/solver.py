@@ -263,6 +263,36 @@ class Solver:
             out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
         return out
 
+    def step(self, Q, time=0.0, graph=True):
+        """The reference driver's step, max_dt then advance (examples/channel_wmles.py:
+        124-125), with dt kept on the device.  graph=True replays the whole launch
+        sequence as one captured hipGraph when the state pointers repeat (the usual
+        `Q, dt = solver.step(Q)` loop alternates two buffers).  Returns (Q_new, dt)."""
+        Q = self._as_state(Q)
+        self._ensure_gauge(Q)
+        if not hasattr(self, "_stream"):
+            self._stream = torch.cuda.Stream(device=self._dev)
+            self._dt_dev = torch.empty(16, dtype=torch.float64, device=self._dev)
+        cur = torch.cuda.current_stream(self._dev)
+        self._stream.wait_stream(cur)
+        with torch.cuda.stream(self._stream):
+            out = tuple(torch.empty_like(q) for q in Q)
+            rc = _lib.lib.wmx_step(self._plan, _lib.ptr_array(Q), _lib.ptr_array(out),
+                                   C.c_void_p(self._dt_dev.data_ptr()), float(time), int(bool(graph)),
+                                   _lib.stream_ptr(self._stream))
+            _lib.check(rc)
+            dt = self._dt_dev[0].clone()
+        cur.wait_stream(self._stream)
+        for q in Q:
+            q.record_stream(self._stream)
+        for c in self._py_constraints:
+            out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
+        return out, dt
+
+    def set_tiling(self, ci_len=16, order=0):
+        """Flux-kernel tiling: i-planes per marching tile, XCD-interleaved order (1) or not."""
+        _lib.check(_lib.lib.wmx_plan_set_tiling(self._plan, int(ci_len), int(order)))
+
     # ------------------------------------------------------------------ diagnostics
     def wall_stats(self, Q):
         """{(block, face): (tau_w, q_w)} for all wall-modelled faces (solver.py:163-179)."""
