@@ -1,14 +1,26 @@
 """Throughput of the explicit SSP-RK3 step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fp64] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form both|fp64|fp32]
+                    [--workload channel256|ramp512|ensemble16|wm512] [--no-cpu-baseline]
 
-Workload: compressible wall-modelled channel, Ma_b 0.3, Re_b 2e4, one block of
-256 x 128 x 128 cells (x, y, z), tanh-clustered y, Hybrid(KEP(4), MUSCL-HLLC) with the
-Ducros switch, Vreman SGS, isothermal equilibrium wall model on both walls, periodic
-x / z, constant mass flux, CFL 0.8, seeded synthetic initial state (bulk profile +
-5% Gaussian velocity noise).  A step is the reference's driver step: max_dt + advance
-(3 RHS + 3 stage updates + constraint), dt kept on the device.  The timed region
-holds exactly K steps bracketed by device synchronisations.
+Default workload (channel256): compressible wall-modelled channel, Ma_b 0.3, Re_b 2e4,
+one block of 256 x 128 x 128 cells (x, y, z), tanh-clustered y, Hybrid(KEP(4),
+MUSCL-HLLC) with the Ducros switch, Vreman SGS, isothermal equilibrium wall model on
+both walls, periodic x / z, constant mass flux, CFL 0.8, seeded synthetic initial state
+(bulk profile + 5% Gaussian velocity noise).  A step is the reference's driver step:
+max_dt + advance (3 RHS + 3 stage updates + constraint), dt kept on the device.  The
+timed region holds exactly K steps bracketed by device synchronisations.  Both
+arithmetic forms run; the FP32-bulk form is the headline value, the FP64 form and the
+FP32-bulk errors (u_tau, mean u(y), q_w against FP64 after the same steps) are reported
+beside it.
+
+Other workloads (one JSON line each):
+  ramp512     smooth-ramp curvilinear block 512 x 128 x 64, inflow + sponge, FP32-bulk
+  ensemble16  16 uneven multiblock channel cases in one plan (one launch per kernel
+              class and stage), graph-captured step, autotuned tiling, against the
+              same cases stepped one after another
+  wm512       channel 512 x 256 x 256 at Re_b 1e5 with the learned wall model (trained
+              here on the GPU) against the Newton model: wall-kernel time, u_tau error
 
 Printed (one JSON line): Mcell-updates/s, ms/step, the roofline of the dominant
 kernel (fused flux / divergence / stage update) from live HIP events, and the CPU
@@ -115,24 +127,24 @@ def form_errors(r64, r32):
             "q_w": float(abs(qw(st32) - qw(st64)) / abs(qw(st64)))}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--form", default="both", choices=["both", "fp64", "fp32"])
-    ap.add_argument("--size", type=int, nargs=3, default=(256, 128, 128))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--breakdown", action="store_true", help="time every kernel class (extra events)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs the step on a single MI355X "
-              "(no multi-GPU path); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def _roofline(avg_launch_s, b_launch, workload):
+    achieved = b_launch / avg_launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(workload),
+            "kernel": "k_flux (fused face fluxes + divergence + RK stage)",
+            "alg_bytes_per_launch": b_launch, "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
 
-    import torch
-    import wmles_jax_amd as wmx
-    torch.cuda.set_device(0)
+
+def _record(args, value, ms_step, dtype, workload, cfg, roofline):
+    return {
+        "metric": "Mcell-updates/s per RK3 step, FP32-bulk vs FP64 at stated error; % HBM peak",
+        "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+        "config": dict({"workload": workload}, **cfg), "roofline": roofline}
+
+
+def bench_channel(args, wmx):
     nx, ny, nz = args.size
     cells = nx * ny * nz
     forms = ["fp64", "fp32"] if args.form == "both" else [args.form]
@@ -140,36 +152,16 @@ def main():
     head = "fp32" if "fp32" in res else "fp64"
     r = res[head]
     el = r["elapsed"]
-    ms_step = el * 1e3 / args.steps
     value = cells * args.steps / el / 1e6
     flux_ms, flux_n = r["prof"]["flux"]
-    avg_flux_s = flux_ms / flux_n / 1e3
     b_launch = B_ALG_STEP[head] / 3.0 * cells
-    achieved = b_launch / avg_flux_s / 1e9
     workload = f"channel {nx}x{ny}x{nz} Hybrid(KEP4+MUSCL-HLLC) Vreman WM {head}"
-    rec = {
-        "metric": "Mcell-updates/s per RK3 step, FP32-bulk vs FP64 at stated error; % HBM peak",
-        "value": round(value, 2),
-        "unit": "Mcell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32-bulk+f64" if head == "fp32" else "f64",
-        "data": "synthetic",
-        "config": {"workload": workload, "cells": cells, "blocks": 1, "ng": 3,
-                   "step": "max_dt + advance (SSP-RK3, 3 RHS, mass-flux constraint)",
-                   "form": head},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profile(workload),
-                     "kernel": "k_flux (fused face fluxes + divergence + RK stage)",
-                     "alg_bytes_per_launch": b_launch, "avg_launch_ms": round(avg_flux_s * 1e3, 4)},
-        "step_hbm_frac": round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
-    }
+    rec = _record(args, value, el * 1e3 / args.steps, "f32-bulk+f64" if head == "fp32" else "f64",
+                  workload, {"cells": cells, "blocks": 1, "ng": 3,
+                             "step": "max_dt + advance (SSP-RK3, 3 RHS, mass-flux constraint)",
+                             "form": head},
+                  _roofline(flux_ms / flux_n / 1e3, b_launch, workload))
+    rec["step_hbm_frac"] = round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
     if "fp64" in res and "fp32" in res:
         v64 = cells * args.steps / res["fp64"]["elapsed"] / 1e6
         rec["fp64_value"] = round(v64, 2)
@@ -181,6 +173,183 @@ def main():
                                      for f in res}
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline()
+    return rec
+
+
+def _timed_steps(step, Q, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        Q = step(Q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        Q = step(Q)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, Q
+
+
+def bench_ramp(args, wmx):
+    """Smooth-ramp curvilinear block 512 x 128 x 64, FP32-bulk."""
+    import torch
+    import cases
+    nx, ny, nz = 512, 128, 64
+    spec, arrays, prim = cases.ramp_spec(wmx, nx, ny, nz)
+    mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": "fp32"})
+    Q = s.init_state(prim)
+    t = [0.0]
+
+    def step(Q):
+        dt = s.max_dt(Q, t[0])
+        Q = s.advance(Q, dt, t[0])
+        return Q
+
+    s.profile(("flux",))
+    el, Q = _timed_steps(step, Q, args.steps, args.warmup)
+    prof = s.profile_read()
+    s.profile(())
+    cells = nx * ny * nz
+    finite = all(bool(torch.isfinite(q).all()) for q in Q)
+    workload = f"smooth ramp {nx}x{ny}x{nz} Hybrid(KEP4+MUSCL-HLLC) Vreman WM inflow+sponge fp32"
+    flux_ms, flux_n = prof["flux"]
+    rec = _record(args, cells * args.steps / el / 1e6, el * 1e3 / args.steps, "f32-bulk+f64", workload,
+                  {"cells": cells, "blocks": 1, "ng": 3, "form": "fp32"},
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
+    rec["state_finite"] = finite
+    return rec
+
+
+def bench_ensemble(args, wmx):
+    """16 uneven multiblock channel cases: one merged plan against sequential cases."""
+    import torch
+    import cases
+    from wmles_jax_amd import autotune
+    specs = cases.ensemble_specs(16)
+    solvers, fns = [], []
+    for spec, arrays, pr, seed in specs:
+        _, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": "fp32"})
+        solvers.append(s)
+        fns.append(lambda xc, pr=pr, seed=seed: cases.channel_prim(xc, pr, seed))
+    cells = sum(int(b.n[0] * b.n[1] * b.n[2]) for s in solvers for b in s.blocks)
+    nblk = sum(len(s.blocks) for s in solvers)
+    ens = wmx.EnsembleSolver(solvers)
+    Qs0 = ens.init_state(fns)
+    best, timings = autotune.tune(ens, Qs0, steps=3, use_cache=False)
+
+    def ens_step(Qs):
+        return ens.step(Qs, graph=True)[0]
+
+    el_e, Qe = _timed_steps(ens_step, Qs0, args.steps, args.warmup)
+    # flux launch time from events on eager (non-replayed) steps of the same plan
+    ens.profile(("flux",))
+    Qp = Qs0
+    for _ in range(5):
+        Qp = ens.step(Qp, graph=False)[0]
+    prof = ens.profile_read()
+    ens.profile(())
+    # the same cases stepped one after another (graph-captured per case), with the
+    # ensemble's tiling and FP32-bulk gauge so the results can be compared bitwise
+    for s in solvers:
+        s.set_tiling(*best)
+        s.set_gauge(*ens._gauge)
+
+    def seq_step(Qs):
+        return [s.step(Q, graph=True)[0] for s, Q in zip(solvers, Qs)]
+
+    el_s, Qseq = _timed_steps(seq_step, Qs0, args.steps, args.warmup)
+    same = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qseq) for a, b in zip(Qa, Qb))
+    workload = f"ensemble of 16 uneven channel cases ({nblk} blocks) Hybrid(KEP4+MUSCL-HLLC) Vreman WM fp32"
+    flux_ms, flux_n = prof["flux"]
+    rec = _record(args, cells * args.steps / el_e / 1e6, el_e * 1e3 / args.steps, "f32-bulk+f64", workload,
+                  {"cells": cells, "cases": 16, "blocks": nblk, "form": "fp32",
+                   "tiling": {"ci_len": best[0], "xcd_order": best[1]}},
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
+    rec["sequential_value"] = round(cells * args.steps / el_s / 1e6, 2)
+    rec["ensemble_speedup_vs_sequential"] = round(el_s / el_e, 3)
+    rec["ensemble_equals_sequential"] = same
+    rec["autotune_flux_ms"] = {k: round(v, 4) for k, v in timings.items()}
+    return rec
+
+
+def bench_wm(args, wmx):
+    """Learned wall model in the 512 x 256 x 256 channel at Re_b 1e5."""
+    import numpy as np
+    import torch
+    import cases
+    nx, ny, nz = 512, 256, 256
+    model = wmx.LearnedWallModel()
+    hist = model.train()
+    spec, arrays = cases.channel_spec(1, nx, ny, nz, {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}},
+                                      "vreman", T_wall=300.0, Re_b=1e5, name=f"channel_{nx}x{ny}x{nz}")
+    mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": "fp32"})
+    pr = spec["props"]
+    Q0 = s.init_state(lambda xc: cases.channel_prim(xc, pr, 0))
+    rho_w = pr["p0"] / (287.05 * 300.0)
+
+    def run(net):
+        s.set_wall_net(net)
+        st0 = s.wall_stats(Q0)
+        s.profile(("wall", "flux"))
+        el, Q = _timed_steps(lambda Q: s.advance(Q, s.max_dt(Q)), Q0, args.steps, args.warmup)
+        prof = s.profile_read()
+        s.profile(())
+        return el, Q, prof, st0
+
+    el_n, Qn, prof_n, st_n = run(None)
+    model.fallbacks.zero_()
+    el_l, Ql, prof_l, st_l = run(model)
+    n_faces = sum(int(v[0].size) for v in st_l.values())
+    launches = prof_l["wall"][1]
+    fb = int(model.fallbacks.item())
+    # same state: per-face tau_w of the two models; after the steps: wall-mean u_tau
+    d_tau = max(float(np.max(np.abs(st_l[k][0] - st_n[k][0]) / np.abs(st_n[k][0]))) for k in st_n)
+    st_ln = s.wall_stats(Ql)
+    s.set_wall_net(None)
+    st_nn = s.wall_stats(Qn)
+    ut = lambda st: float(np.sqrt(np.mean([float(v[0].mean()) for v in st.values()]) / rho_w))
+    cells = nx * ny * nz
+    workload = f"channel {nx}x{ny}x{nz} Re_b 1e5 Hybrid(KEP4+MUSCL-HLLC) Vreman learned WM fp32"
+    flux_ms, flux_n = prof_l["flux"]
+    rec = _record(args, cells * args.steps / el_l / 1e6, el_l * 1e3 / args.steps, "f32-bulk+f64", workload,
+                  {"cells": cells, "blocks": 1, "form": "fp32", "wall_faces": n_faces,
+                   "network": "3-64-64-2 tanh, BF16 MFMA inference, 3 Newton polish steps"},
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
+    rec["wall_model"] = {
+        "newton_ms_per_launch": round(prof_n["wall"][0] / prof_n["wall"][1], 5),
+        "learned_ms_per_launch": round(prof_l["wall"][0] / prof_l["wall"][1], 5),
+        "speedup": round((prof_n["wall"][0] / prof_n["wall"][1]) / (prof_l["wall"][0] / prof_l["wall"][1]), 3),
+        "fallback_fraction": round(fb / max(1, launches * n_faces), 6),
+        "tau_w_max_rel_err_same_state": d_tau,
+        "u_tau_rel_err_after_steps": abs(ut(st_ln) - ut(st_nn)) / ut(st_nn),
+        "newton_value": round(cells * args.steps / el_n / 1e6, 2),
+        "train_s": round(hist["train_s"], 3), "data_s": round(hist["data_s"], 3),
+        "train_steps": hist["steps"], "train_samples": hist["n_samples"], "final_mse": hist["final_mse"]}
+    return rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--form", default="both", choices=["both", "fp64", "fp32"])
+    ap.add_argument("--workload", default="channel256", choices=["channel256", "ramp512", "ensemble16", "wm512"])
+    ap.add_argument("--size", type=int, nargs=3, default=(256, 128, 128))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="time every kernel class (extra events)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the step on a single MI355X "
+              "(no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.steps is None:
+        args.steps = {"channel256": 200, "ramp512": 200, "ensemble16": 100, "wm512": 20}[args.workload]
+
+    import torch
+    import wmles_jax_amd as wmx
+    torch.cuda.set_device(0)
+    fn = {"channel256": bench_channel, "ramp512": bench_ramp, "ensemble16": bench_ensemble,
+          "wm512": bench_wm}[args.workload]
+    rec = fn(args, wmx)
     print(json.dumps(rec), flush=True)
 
 

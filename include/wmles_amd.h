@@ -117,6 +117,27 @@ typedef struct wmx_inflow_desc {
   double U_inf, L, rho_inf, p_inf;
 } wmx_inflow_desc;
 
+/* Learned wall model (no reference code: it plugs in at the wall-model callable
+   boundary model(gas, h, U, T_h, p_h, T_wall), reference wmles/wall_model.py:3-9,
+   called at wmles/bc.py:161).  MLP 3 -> 64 -> 64 -> 2 with tanh hidden layers;
+   inputs log Re_h, h/Delta, T_h/T_w; outputs log u+ and log T+ (normalised).
+   FP32 master parameters (wmx_mlp_sizes: nparam) are ordered
+   W1[64][3] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]; wmx_mlp_pack derives the
+   inference copies w_bf16 = W2 | W3 (BF16 bits) and w_fp32 = W1 | b1 | b2 | b3. */
+typedef struct wmx_wall_net_desc {
+  const uint16_t* w_bf16;   /* device, nbf16 entries                                */
+  const float* w_fp32;      /* device, nfp32 entries                                */
+  float x_center[3], x_scale[3];   /* x' = (x - center) / scale                    */
+  float y_center[2], y_scale[2];   /* log y = y' scale + center                    */
+  double lo[3], hi[3];      /* trained range of (log Re_h, h/Delta, T_h/T_w); a point
+                               outside the log Re_h or T_h/T_w range falls back to
+                               the full Newton solve; h/Delta is clamped (the
+                               equilibrium law does not depend on it)             */
+  int32_t n_polish;         /* Newton steps from the network's u+                   */
+  double tol;               /* fall back when |F(u+)| / u+ > tol after the polish   */
+  uint32_t* fallback_count; /* optional device counter, incremented per fallback    */
+} wmx_wall_net_desc;
+
 typedef struct wmx_plan wmx_plan;
 
 /* Last error message of the calling thread ("" if none). */
@@ -215,6 +236,42 @@ int wmx_fill_face(const wmx_block_desc* block, int32_t axis, int32_t side,
 int wmx_wall_model_eval(const wmx_numerics_desc* numerics, int32_t mode, int64_t n,
                         const double* a, const double* b, const double* c, const double* d,
                         double T_wall, double* out0, double* out1, void* stream);
+
+/* ---- learned wall model (no counterpart in the reference; see wmx_wall_net_desc) ---- */
+/* Parameter counts: FP32 master, BF16 inference copy, FP32 inference copy. */
+int wmx_mlp_sizes(int32_t* nparam, int32_t* nbf16, int32_t* nfp32);
+/* n training samples of the method: features uniform over net->lo..hi, u+ by the
+   Newton of wmles/wall_model.py:19-40, T+ by Kader (wall_model.py:56-60).  X (n, 3)
+   and Y (n, 2) normalised with net's centers/scales; raw (n, 5) optional:
+   log Re_h, h/Delta, T_h/T_w, u+, T+.  Deterministic in (seed, index). */
+int wmx_wm_dataset(const wmx_numerics_desc* numerics, const wmx_wall_net_desc* net, int64_t n,
+                   uint64_t seed, float* X, float* Y, double* raw, void* stream);
+/* One FP32 forward + backward over samples [offset, offset + batch) of (X, Y)
+   (mean squared error over batch x 2 outputs).  partials: ceil(batch/64) * nparam
+   floats; loss_partials: ceil(batch/64) sums of squared errors.  grad (optional):
+   the summed gradient.  update != 0: Adam step `step` (>= 1) with rate lr on params. */
+int wmx_mlp_train_step(float* params, const float* X, const float* Y, int64_t n_samples,
+                       int64_t offset, int32_t batch, float* partials, float* loss_partials,
+                       float* grad, float* adam_m, float* adam_v, int32_t step, float lr,
+                       int32_t update, void* stream);
+/* FP32 master -> inference copies (BF16 W2 | W3, FP32 W1 | b1 | b2 | b3). */
+int wmx_mlp_pack(const float* params, uint16_t* w_bf16, float* w_fp32, void* stream);
+/* FP32 forward of the master parameters on n normalised inputs -> Y (n, 2). */
+int wmx_mlp_forward(const float* params, const float* X, int64_t n, float* Y, void* stream);
+/* The learned model on n points: the wall-model callable of wmles/wall_model.py:42-61
+   with the Newton start taken from the BF16 MFMA network.  mode 0: n_polish Newton
+   steps, residual check, Newton fallback, q_w by Kader at the polished y+; mode 1:
+   network outputs only (u+ and T+).  hdel optional (h/Delta).  uplus optional. */
+int wmx_wall_net_eval(const wmx_numerics_desc* numerics, const wmx_wall_net_desc* net,
+                      int32_t mode, int64_t n, const double* h, const double* hdel,
+                      const double* U, const double* T_h, const double* p_h, double T_wall,
+                      double* tau_w, double* q_w, double* uplus, void* stream);
+/* Use the learned model (mode 0) in the plan's wall-flux kernel; net = NULL restores
+   the Newton model.  The descriptor's device buffers must outlive the plan's use. */
+int wmx_plan_set_wall_net(wmx_plan* plan, const wmx_wall_net_desc* net);
+/* D = A B for one 16x16x16 tile through v_mfma_f32_16x16x16bf16_1k with the lane
+   layout the network kernels use (A, B row-major FP32, rounded to BF16; D FP32). */
+int wmx_mfma_bf16_probe(const float* A, const float* B, float* D, void* stream);
 
 #ifdef __cplusplus
 }

@@ -157,6 +157,92 @@ def channel_prim(xc, props, seed=0, noise=0.05, interior=None):
     return np.stack([rho, vel[0], vel[1], vel[2], p])
 
 
+def ramp_spec(ns, nx=512, ny=128, nz=64, first_dy=1e-3, n_modes=64, seed=3, cfl=0.5):
+    """Synthetic smooth-ramp block: x/L in [-0.76, 4], wall from y/L = 0.22 to 0 by a
+    quintic over x/L in [0, 1], top at y/L = 0.74, one-sided tanh wall-normal stretching
+    with first spacing `first_dy` L at the inflow, periodic span 0.288 L; Ma 0.2,
+    Re_L 667000; 1/7-power inflow with Fourier-mode fluctuations of 8% U rms;
+    wall-modelled (adiabatic) bottom, slip top, characteristic outflow, sponge over the
+    last 1 L.  Returns (spec, arrays, prim_fn)."""
+    props = _resolve(ns, "flow_properties")(0.2, 667000.0)
+    Lr = props["L"]
+    x = np.linspace(-0.76, 4.0, nx + 1)
+    xi = np.clip(x, 0.0, 1.0)
+    wall = 0.22 * (1.0 - (10 * xi ** 3 - 15 * xi ** 4 + 6 * xi ** 5))
+    eta = np.linspace(0.0, 1.0, ny + 1)
+
+    def st(beta):
+        return 1.0 + np.tanh(beta * (eta - 1.0)) / np.tanh(beta)
+
+    lo, hi = 1e-3, 20.0                         # first spacing decreases with beta
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if st(mid)[1] * 0.52 > first_dy:
+            lo = mid
+        else:
+            hi = mid
+    s_eta = st(0.5 * (lo + hi))
+    y = wall[:, None] + (0.74 - wall[:, None]) * s_eta[None, :]
+    z = np.linspace(0.0, 0.288, nz + 1)
+    shp = (nx + 1, ny + 1, nz + 1)
+    nodes = np.stack([np.broadcast_to(x[:, None, None], shp), np.broadcast_to(y[:, :, None], shp),
+                      np.broadcast_to(z[None, None, :], shp)]) * Lr
+    yt = 0.22 + 0.52 * np.linspace(0.0, 1.0, 64) ** 2
+    U = np.minimum(((yt - 0.22) / 0.028) ** (1.0 / 7.0), 1.0)
+    a = 0.08 ** 2
+    prof = np.stack([yt, U, 0.0 * yt, a + 0 * yt, a + 0 * yt, a + 0 * yt, -0.3 * a + 0 * yt], axis=1)
+    prim_inf = [props["rho_inf"], props["U_inf"], 0.0, 0.0, props["p_inf"]]
+    rp = {k: float(v) for k, v in props.items() if k != "gas"}
+    spec = {"name": f"ramp_{nx}x{ny}x{nz}", "blocks": ["ramp"], "ng": 3,
+            "connect": [["ramp", "kmin", "ramp", "kmax", [0.0, 0.0, -0.288 * Lr]]],
+            "bcs": [["ramp", "imin", {"kind": "SmoothRampInflow", "length_scales": [0.1, 0.02, 0.03],
+                                      "seed": seed, "n_modes": n_modes, "span_cells": nz}],
+                    ["ramp", "imax", {"kind": "CharacteristicFarfield", "prim": prim_inf}],
+                    ["ramp", "jmin", {"kind": "WallModel", "T_wall": None, "n_match": 1}],
+                    ["ramp", "jmax", {"kind": "SlipWall"}]],
+            "scheme": {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}},
+            "sgs": "vreman", "sgs_kwargs": {}, "cfl": cfl, "constraints": [],
+            "sources": [{"kind": "SmoothRampSponge", "x_start_l": 3.0, "x_end_l": 4.0,
+                         "strength": 1.0}],
+            "gas": {}, "ramp_props": rp}
+    arrays = {"nodes_ramp": nodes, "inflow_profile": prof}
+
+    def prim(xc):
+        x_l = xc[0] / Lr
+        xx = np.clip(x_l, 0.0, 1.0)
+        wl = 0.22 * (1.0 - (10 * xx ** 3 - 15 * xx ** 4 + 6 * xx ** 5))
+        yd = xc[1] / Lr - wl + 0.22
+        u = props["U_inf"] * np.interp(yd, prof[:, 0], prof[:, 1])
+        zero = 0.0 * u
+        return np.stack([props["rho_inf"] + zero, u, zero, zero, props["p_inf"] + zero])
+
+    return spec, arrays, prim
+
+
+def ensemble_specs(n_cases=16, seed=2024, scheme=None, sgs="vreman"):
+    """`n_cases` channel cases: Re_b log-uniform in [5e3, 4e4], grids between
+    64x48x32 and 192x96x64 (multiples of 8), each split in x into 2-4 blocks of unequal
+    size.  Returns [(spec, arrays, props, seed)]."""
+    rng = np.random.default_rng(seed)
+    scheme = scheme or {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}}
+    out = []
+    for c in range(n_cases):
+        Re_b = float(np.exp(rng.uniform(np.log(5e3), np.log(4e4))))
+        nx = int(rng.integers(8, 25)) * 8
+        ny = int(rng.integers(6, 13)) * 8
+        nz = int(rng.integers(4, 9)) * 8
+        nb = int(rng.integers(2, 5))
+        w = rng.uniform(0.5, 1.5, nb)
+        cuts = np.concatenate([[0.0], np.cumsum(w) / w.sum()]) * nx
+        cuts = [0] + [int(round(v)) for v in cuts[1:-1]] + [nx]
+        for i in range(1, nb):
+            cuts[i] = min(max(cuts[i], cuts[i - 1] + 8), nx - 8 * (nb - i))
+        spec, arrays = channel_spec(nb, nx, ny, nz, scheme, sgs, Re_b=Re_b, splits=cuts,
+                                    name=f"case{c}_{nx}x{ny}x{nz}_{nb}blk")
+        out.append((spec, arrays, spec["props"], c))
+    return out
+
+
 def dumps(spec):
     return json.dumps(spec)
 
@@ -192,7 +278,7 @@ def make_scheme(ns, sc):
 def make_bc(ns, bc, props=None, arrays=None):
     kind = bc["kind"]
     if kind == "WallModel":
-        wm = _resolve(ns, "EquilibriumWallModel")(**bc.get("wm", {}))
+        wm = bc["model"] if "model" in bc else _resolve(ns, "EquilibriumWallModel")(**bc.get("wm", {}))
         return _resolve(ns, "WallModel")(wm, T_wall=bc.get("T_wall"), n_match=bc.get("n_match", 1))
     if kind == "NoSlipWall":
         return _resolve(ns, "NoSlipWall")(bc.get("T_wall"))

@@ -36,6 +36,33 @@ def test_struct_layouts_match_header_sizes():
     assert ctypes.sizeof(_lib.BlockDesc) == 16 + 6 * 120 + 8 * 8
 
 
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """sizeof / offsetof of every field of the ctypes mirrors equal the C compiler's."""
+    import ctypes
+    import subprocess
+    from wmles_jax_amd import _lib
+    structs = {"wmx_face_desc": _lib.FaceDesc, "wmx_block_desc": _lib.BlockDesc,
+               "wmx_numerics_desc": _lib.NumericsDesc, "wmx_inflow_desc": _lib.InflowDesc,
+               "wmx_wall_net_desc": _lib.WallNetDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "wmles_amd.h"', "int main(void) {"]
+    expect = []
+    for cname, py in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(ctypes.sizeof(py))
+        for fname, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append(getattr(py, fname).offset)
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert got == expect
+
+
 @pytest.mark.parametrize("kind", ["split", "flip", "rot_cyc"])
 def test_orientation_matching_equals_oracle(kind):
     """Host-side interface orientation (mesh.py:169-193) equals the oracle's."""
@@ -80,3 +107,29 @@ def test_face_descriptor_packing():
     assert fd.kind == 0 and fd.nb == itf.nb and tuple(fd.mapping) == itf.mapping
     wmf = d.faces[0]
     assert wmf.kind == 6 and wmf.T_wall == 300.0 and wmf.n_match == 2
+
+
+def test_ensemble_descriptor_merge_offsets_neighbours():
+    """Merged ensemble plans: case ids and interface neighbours shifted per case."""
+    from wmles_jax_amd import _lib
+    from wmles_jax_amd.ensemble import merge_block_descs
+
+    def case(nblk):
+        arr = (_lib.BlockDesc * nblk)()
+        for b in range(nblk):
+            for f in range(6):
+                arr[b].faces[f].kind = _lib.BC_SLIP
+            arr[b].faces[0].kind = _lib.FACE_INTERFACE        # imin <- previous block
+            arr[b].faces[0].nb = (b - 1) % nblk
+            arr[b].faces[1].kind = _lib.FACE_INTERFACE        # imax <- next block
+            arr[b].faces[1].nb = (b + 1) % nblk
+            arr[b].n[0] = 10 + b
+        return arr
+
+    descs, offs = merge_block_descs([case(2), case(3), case(1)])
+    assert offs == [0, 2, 5]
+    assert [d.case_id for d in descs] == [0, 0, 1, 1, 1, 2]
+    assert [d.faces[0].nb for d in descs] == [1, 0, 4, 2, 3, 5]
+    assert [d.faces[1].nb for d in descs] == [1, 0, 3, 4, 2, 5]
+    assert [d.n[0] for d in descs] == [10, 11, 10, 11, 12, 10]
+    assert all(d.faces[2].nb == 0 for d in descs)             # non-interface faces untouched

@@ -12,6 +12,7 @@ from .gas import Gas
 from .mesh import Block, Mesh, Geom, Interface, FACE_NAMES, sl
 from .bc import BC, SlipWall, NoSlipWall, Dirichlet, CharacteristicFarfield, Extrapolate, WallModel
 from .wall_model import EquilibriumWallModel
+from .learned_wm import LearnedWallModel
 from .schemes import KEP, KEEP, MUSCLHLLC, Hybrid
 from .sgs import make_sgs
 from .solver import Solver, ConstantMassFlux
@@ -22,6 +23,6 @@ from . import autotune
 
 __all__ = ["HAS_JAX", "Gas", "Block", "Mesh", "Geom", "Interface", "FACE_NAMES", "sl", "BC",
            "SlipWall", "NoSlipWall", "Dirichlet", "CharacteristicFarfield", "Extrapolate",
-           "WallModel", "EquilibriumWallModel", "KEP", "KEEP", "MUSCLHLLC", "Hybrid", "make_sgs",
+           "WallModel", "EquilibriumWallModel", "LearnedWallModel", "KEP", "KEEP", "MUSCLHLLC", "Hybrid", "make_sgs",
            "Solver", "ConstantMassFlux", "smooth_ramp", "SmoothRampInflow", "SmoothRampSponge",
            "flow_properties", "EnsembleSolver", "autotune"]

@@ -52,6 +52,14 @@ class InflowDesc(C.Structure):
                 ("rho_inf", C.c_double), ("p_inf", C.c_double)]
 
 
+class WallNetDesc(C.Structure):
+    _fields_ = [("w_bf16", C.c_void_p), ("w_fp32", C.c_void_p),
+                ("x_center", C.c_float * 3), ("x_scale", C.c_float * 3),
+                ("y_center", C.c_float * 2), ("y_scale", C.c_float * 2),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("n_polish", C.c_int32), ("tol", C.c_double), ("fallback_count", C.c_void_p)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -88,6 +96,17 @@ def _load():
                            C.c_double, P], C.c_int),
         "wmx_wall_model_eval": ([C.POINTER(NumericsDesc), C.c_int32, C.c_int64, P, P, P, P,
                                  C.c_double, P, P, P], C.c_int),
+        "wmx_mlp_sizes": ([C.POINTER(C.c_int32)] * 3, C.c_int),
+        "wmx_wm_dataset": ([C.POINTER(NumericsDesc), C.POINTER(WallNetDesc), C.c_int64, C.c_uint64,
+                            P, P, P, P], C.c_int),
+        "wmx_mlp_train_step": ([P, P, P, C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P, C.c_int32,
+                                C.c_float, C.c_int32, P], C.c_int),
+        "wmx_mlp_pack": ([P, P, P, P], C.c_int),
+        "wmx_mlp_forward": ([P, P, C.c_int64, P, P], C.c_int),
+        "wmx_wall_net_eval": ([C.POINTER(NumericsDesc), C.POINTER(WallNetDesc), C.c_int32, C.c_int64,
+                               P, P, P, P, P, C.c_double, P, P, P, P], C.c_int),
+        "wmx_plan_set_wall_net": ([P, C.POINTER(WallNetDesc)], C.c_int),
+        "wmx_mfma_bf16_probe": ([P, P, P, P], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -121,5 +140,5 @@ def stream_ptr(stream=None):
     return C.c_void_p(s.cuda_stream)
 
 
-__all__ = ["lib", "check", "WmxError", "FaceDesc", "BlockDesc", "NumericsDesc", "InflowDesc",
+__all__ = ["lib", "check", "WmxError", "FaceDesc", "BlockDesc", "NumericsDesc", "InflowDesc", "WallNetDesc",
            "ptr_array", "stream_ptr", "LIB_PATH"]

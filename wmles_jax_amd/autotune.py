@@ -40,14 +40,13 @@ def tune(obj, Q, steps=3, candidates=CANDIDATES, use_cache=True):
         ci, order = cache[key]["best"]
         obj.set_tiling(ci, order)
         return (ci, order), cache[key]["timings"]
-    plan_owner = getattr(obj, "solvers", None)
     timings = {}
     for ci, order in candidates:
         obj.set_tiling(ci, order)
         Qw = Q
         Qw, _ = obj.step(Qw, graph=False)            # warm-up
         torch.cuda.synchronize()
-        prof = obj if plan_owner is None else None
+        prof = obj if hasattr(obj, "profile") else None
         if prof is not None:
             prof.profile(("flux",))
         t0 = torch.cuda.Event(enable_timing=True)

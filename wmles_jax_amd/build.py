@@ -29,6 +29,7 @@ UNITS = {
     "reduce.hip": ["-ffp-contract=off"],
     "rhs.hip": ["-ffp-contract=fast"],
     "flux.hip": ["-ffp-contract=fast"],
+    "mlp.hip": ["-ffp-contract=fast"],
     "capi.hip": [],
 }
 
@@ -45,7 +46,7 @@ def _compile(unit, flags):
 
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(5, len(UNITS))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(UNITS))) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs]

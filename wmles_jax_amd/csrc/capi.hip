@@ -397,6 +397,43 @@ int check_plan(const wmx_plan* p) {
 
 }  // namespace
 
+void drop_graphs(wmx_plan* p) {
+  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.exec);
+  p->graphs.clear();
+}
+
+int check_net(const wmx_wall_net_desc* n) {
+  if (!n->w_bf16 || !n->w_fp32) return fail("wall net: null weight pointer");
+  for (int k = 0; k < 3; ++k)
+    if (!(n->hi[k] > n->lo[k]) || !(n->x_scale[k] > 0.f)) return fail("wall net: empty input range");
+  for (int k = 0; k < 2; ++k)
+    if (!(n->y_scale[k] > 0.f)) return fail("wall net: y_scale must be > 0");
+  if (n->n_polish < 0 || n->n_polish > 100) return fail("wall net: n_polish must be in [0, 100]");
+  if (!(n->tol > 0.0)) return fail("wall net: tol must be > 0");
+  return 0;
+}
+
+WallNetDev make_netdev(const wmx_wall_net_desc* n) {
+  WallNetDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.w = n->w_bf16;
+  d.f = n->w_fp32;
+  for (int k = 0; k < 3; ++k) {
+    d.xc[k] = n->x_center[k];
+    d.xs[k] = n->x_scale[k];
+    d.lo[k] = n->lo[k];
+    d.hi[k] = n->hi[k];
+  }
+  for (int k = 0; k < 2; ++k) {
+    d.yc[k] = n->y_center[k];
+    d.ys[k] = n->y_scale[k];
+  }
+  d.n_polish = n->n_polish;
+  d.tol = n->tol;
+  d.nfb = n->fallback_count;
+  return d;
+}
+
 extern "C" {
 
 const char* wmx_last_error(void) { return g_err.c_str(); }
@@ -643,6 +680,19 @@ int wmx_plan_set_gauge(wmx_plan* p, double p_ref, double T_ref) {
   p->nd.p_ref = p_ref;
   p->nd.T_gauge = T_ref;
   p->nd.e_ref = p->nd.gas.R * T_ref / (p->nd.gas.gamma - 1.0);
+  drop_graphs(p);   // captured launches hold the old values
+  return 0;
+}
+
+int wmx_plan_set_wall_net(wmx_plan* p, const wmx_wall_net_desc* net) {
+  if (!p) return fail("null plan");
+  if (net) {
+    if (check_net(net)) return 1;
+    p->nd.net = make_netdev(net);
+  } else {
+    std::memset(&p->nd.net, 0, sizeof(p->nd.net));
+  }
+  drop_graphs(p);
   return 0;
 }
 
@@ -806,8 +856,7 @@ int wmx_plan_set_tiling(wmx_plan* p, int32_t ci_len, int32_t order) {
   (void)hipFree(p->d_tiles);
   p->d_tiles = nullptr;
   if (upload(p->tiles, &p->d_tiles)) return 1;
-  for (auto& g : p->graphs) (void)hipGraphExecDestroy(g.exec);
-  p->graphs.clear();
+  drop_graphs(p);
   return 0;
 }
 
@@ -923,6 +972,86 @@ int wmx_wall_model_eval(const wmx_numerics_desc* nm, int32_t mode, int64_t n, co
   if (mode != 0 && mode != 1) return fail("mode must be 0 or 1");
   NumDev nd = make_numdev(nm);
   WMX_CHECK(launch_wm_eval(nd, mode, n, a, b, c, d, T_wall, o0, o1, as_stream(stream)));
+  return 0;
+}
+
+int wmx_mlp_sizes(int32_t* nparam, int32_t* nbf16, int32_t* nfp32) {
+  if (!nparam || !nbf16 || !nfp32) return fail("null argument");
+  *nparam = mlp_nparam();
+  *nbf16 = mlp_nbf16();
+  *nfp32 = mlp_nf32();
+  return 0;
+}
+
+int wmx_wm_dataset(const wmx_numerics_desc* nm, const wmx_wall_net_desc* net, int64_t n, uint64_t seed,
+                   float* X, float* Y, double* raw, void* stream) {
+  if (!nm || !net || !X || !Y) return fail("null argument");
+  if (n < 0) return fail("n must be >= 0");
+  for (int k = 0; k < 3; ++k)
+    if (!(net->hi[k] >= net->lo[k]) || !(net->x_scale[k] > 0.f)) return fail("wall net: bad input range");
+  for (int k = 0; k < 2; ++k)
+    if (!(net->y_scale[k] > 0.f)) return fail("wall net: y_scale must be > 0");
+  NumDev nd = make_numdev(nm);
+  WallNetDev d;
+  std::memset(&d, 0, sizeof(d));
+  for (int k = 0; k < 3; ++k) {
+    d.xc[k] = net->x_center[k];
+    d.xs[k] = net->x_scale[k];
+    d.lo[k] = net->lo[k];
+    d.hi[k] = net->hi[k];
+  }
+  for (int k = 0; k < 2; ++k) {
+    d.yc[k] = net->y_center[k];
+    d.ys[k] = net->y_scale[k];
+  }
+  nd.net = d;
+  WMX_CHECK(launch_wm_dataset(nd, n, seed, X, Y, raw, as_stream(stream)));
+  return 0;
+}
+
+int wmx_mlp_train_step(float* params, const float* X, const float* Y, int64_t n_samples, int64_t offset,
+                       int32_t batch, float* partials, float* loss_partials, float* grad, float* adam_m,
+                       float* adam_v, int32_t step, float lr, int32_t update, void* stream) {
+  if (!params || !X || !Y || !partials || !loss_partials) return fail("null argument");
+  if (batch < 1) return fail("batch must be >= 1");
+  if (offset < 0 || offset + batch > n_samples) return fail("batch [offset, offset + batch) exceeds the samples");
+  if (update && (!adam_m || !adam_v || step < 1)) return fail("update needs adam_m, adam_v and step >= 1");
+  WMX_CHECK(launch_mlp_train(params, X, Y, offset, batch, partials, loss_partials, grad, adam_m, adam_v,
+                             step, lr, update, as_stream(stream)));
+  return 0;
+}
+
+int wmx_mlp_pack(const float* params, uint16_t* w_bf16, float* w_fp32, void* stream) {
+  if (!params || !w_bf16 || !w_fp32) return fail("null argument");
+  WMX_CHECK(launch_mlp_pack(params, w_bf16, w_fp32, as_stream(stream)));
+  return 0;
+}
+
+int wmx_mlp_forward(const float* params, const float* X, int64_t n, float* Y, void* stream) {
+  if (!params || !X || !Y) return fail("null argument");
+  if (n < 0) return fail("n must be >= 0");
+  WMX_CHECK(launch_mlp_forward32(params, X, n, Y, as_stream(stream)));
+  return 0;
+}
+
+int wmx_wall_net_eval(const wmx_numerics_desc* nm, const wmx_wall_net_desc* net, int32_t mode, int64_t n,
+                      const double* h, const double* hdel, const double* U, const double* T_h,
+                      const double* p_h, double T_wall, double* tau_w, double* q_w, double* uplus,
+                      void* stream) {
+  if (!nm || !net || !h || !U || !T_h || !p_h || !tau_w || !q_w) return fail("null argument");
+  if (mode != 0 && mode != 1) return fail("mode must be 0 or 1");
+  if (n < 0) return fail("n must be >= 0");
+  if (check_net(net)) return 1;
+  NumDev nd = make_numdev(nm);
+  nd.net = make_netdev(net);
+  WMX_CHECK(launch_wm_net_eval(nd, mode, n, h, hdel, U, T_h, p_h, T_wall, tau_w, q_w, uplus,
+                               as_stream(stream)));
+  return 0;
+}
+
+int wmx_mfma_bf16_probe(const float* A, const float* B, float* D, void* stream) {
+  if (!A || !B || !D) return fail("null argument");
+  WMX_CHECK(launch_mfma_probe(A, B, D, as_stream(stream)));
   return 0;
 }
 

@@ -40,6 +40,18 @@ struct BlockDev {
   wmx_face_desc faces[6];
 };
 
+// Learned wall model handed to the wall kernels; w == nullptr selects the Newton model.
+struct WallNetDev {
+  const unsigned short* w;  // BF16 W2 [64][64] | W3 [2][64]
+  const float* f;           // FP32 W1 [64][3] | b1 [64] | b2 [64] | b3 [2]
+  float xc[3], xs[3];       // input normalisation x' = (x - xc) / xs
+  float yc[2], ys[2];       // output de-normalisation log y = y' ys + yc
+  double lo[3], hi[3];      // trained range of (log Re_h, h/Delta, T_h/T_w)
+  int n_polish;             // Newton steps from the network's u+
+  double tol;               // fallback when |F(u+)| / u+ exceeds it
+  unsigned int* nfb;        // optional fallback counter
+};
+
 struct NumDev {
   GasC gas;
   WallModelC wm;
@@ -53,6 +65,7 @@ struct NumDev {
   int prec;                 // WMX_PREC_FP64 / WMX_PREC_FP32_BULK
   double p_ref, T_gauge, e_ref;   // FP32-bulk gauge: p' = p - p_ref, T' = T - T_gauge,
                                   // e' = e_int - e_ref with e_ref = R T_gauge / (gamma - 1)
+  WallNetDev net;
 };
 
 struct GhostOp {
@@ -107,6 +120,22 @@ hipError_t launch_mf_apply(const BlockDev* d_blks, const int* d_pre, int nblk, i
 hipError_t launch_wm_eval(const NumDev& nd, int mode, long long n, const double* a, const double* b,
                           const double* c, const double* d, double T_wall, double* o0, double* o1,
                           hipStream_t s);
+// learned wall model (mlp.hip)
+hipError_t launch_wm_dataset(const NumDev& nd, long long n, unsigned long long seed, float* X,
+                             float* Y, double* raw, hipStream_t s);
+hipError_t launch_mlp_train(float* P, const float* X, const float* Y, long long off, int batch,
+                            float* gpart, float* lpart, float* gout, float* m, float* v, int step,
+                            float lr, int update, hipStream_t s);
+hipError_t launch_mlp_pack(const float* P, unsigned short* W, float* F, hipStream_t s);
+hipError_t launch_mlp_forward32(const float* P, const float* X, long long n, float* Y, hipStream_t s);
+hipError_t launch_wm_net_eval(const NumDev& nd, int mode, long long n, const double* h,
+                              const double* hdel, const double* U, const double* Th,
+                              const double* ph, double T_wall, double* tau, double* qw,
+                              double* uplus, hipStream_t s);
+hipError_t launch_mfma_probe(const float* A, const float* B, float* D, hipStream_t s);
+int mlp_nparam();
+int mlp_nbf16();
+int mlp_nf32();
 hipError_t launch_sponge_mean(const BlockDev* d_blks, int blk, long long rows, Ptrs q,
                               hipStream_t s);
 

@@ -4,9 +4,11 @@
 //   k_grad     cells [ng-1, ng+n] per axis: Green-Gauss grad of [u v w T], SGS mu_t,
 //              Ducros theta, mu_eff, k_eff                            (viscous.py:6-25,
 //                                                                       sgs.py, solver.py:78-85)
-//   k_wall     WallModel faces: wall flux, tau_w, q_w                 (bc.py:145-166)
+//   k_wall     WallModel faces: wall flux, tau_w, q_w                 (bc.py:145-166),
+//              with the Newton model or the learned model of wallnet.h
 //   (the fused face-flux / divergence / stage kernel is in flux.hip)
 #include "common.h"
+#include "wallnet.h"
 
 namespace wmx {
 
@@ -145,11 +147,23 @@ template __global__ void k_grad<double>(const BlockDev*, const int*, int, NumDev
 template __global__ void k_grad<float>(const BlockDev*, const int*, int, NumDev);
 
 // ------------------------------------------------------------------------------ wall
+// NET: the learned model (wave-cooperative network + Newton polish) replaces the
+// Newton solve; whole waves past the face count leave, the others clamp their index.
+template <bool NET>
 __global__ __launch_bounds__(256) void k_wall(const BlockDev* __restrict__ blks,
                                               const WallOp* __restrict__ ops, Ptrs q, NumDev nd) {
+  __shared__ unsigned short sW[NET ? MLP_LDS_BF16 : 1];
+  __shared__ float sF[NET ? MLP_NF32 : 1];
   const WallOp op = ops[blockIdx.y];
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= op.count) return;
+  const int idx0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (NET) {
+    net_stage(nd.net, sW, sF);
+    __syncthreads();
+    if ((int)(blockIdx.x * blockDim.x + (threadIdx.x & ~63)) >= op.count) return;
+  } else if (idx0 >= op.count) {
+    return;
+  }
+  const int idx = NET ? min(idx0, op.count - 1) : idx0;
   const BlockDev& B = blks[op.blk];
   const int a = op.axis, s = op.side, ng = B.ng, n = B.n[a];
   const wmx_face_desc& f = B.faces[2 * a + s];
@@ -191,7 +205,16 @@ __global__ __launch_bounds__(256) void k_wall(const BlockDev* __restrict__ blks,
   double up[3] = {wm.u - un * no[0], wm.v - un * no[1], wm.w - un * no[2]};
   double U = sqrt(((up[0] * up[0] + up[1] * up[1]) + up[2] * up[2]) + 1e-20);
   double tau, qw;
-  equilibrium_wall(gas, nd.wm, h, U, wm.T, wm.p, f.T_wall, &tau, &qw);
+  if (NET) {
+    double up;
+    const bool fb = learned_wall(nd, sW, sF, 0, h, h / cbrt(B.vol[m]), U, wm.T, wm.p, f.T_wall,
+                                 &tau, &qw, &up);
+    const unsigned long long bal = __ballot(fb && idx0 < op.count);
+    if (nd.net.nfb && (threadIdx.x & 63) == 0 && bal) atomicAdd(nd.net.nfb, (unsigned int)__popcll(bal));
+    if (idx0 >= op.count) return;
+  } else {
+    equilibrium_wall(gas, nd.wm, h, U, wm.T, wm.p, f.T_wall, &tau, &qw);
+  }
   double sg = s == 1 ? 1.0 : -1.0;
   double* W = B.wall[2 * a + s];
   const long long nw = (long long)B.n[t1] * B.n[t2];
@@ -300,7 +323,10 @@ hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, in
                        const NumDev& nd, hipStream_t s) {
   if (nops <= 0 || maxcount <= 0) return hipSuccess;
   dim3 grid((maxcount + 255) / 256, nops);
-  hipLaunchKernelGGL(k_wall, grid, dim3(256), 0, s, d_blks, d_ops, q, nd);
+  if (nd.net.w)
+    hipLaunchKernelGGL(k_wall<true>, grid, dim3(256), 0, s, d_blks, d_ops, q, nd);
+  else
+    hipLaunchKernelGGL(k_wall<false>, grid, dim3(256), 0, s, d_blks, d_ops, q, nd);
   return hipGetLastError();
 }
 

@@ -13,7 +13,36 @@ import ctypes as C
 import torch
 
 from . import _lib
-from .solver import numerics_desc
+from .solver import numerics_desc, wall_model_of
+
+
+def _numerics_key(nd):
+    return tuple(getattr(nd, f) for f in ("scheme", "central", "order", "limiter", "th0", "th1", "sgs",
+                                          "sgs_c", "cfl", "bc_passes", "precision", "kappa", "B",
+                                          "n_iter"))
+
+
+def merge_block_descs(case_descs):
+    """Concatenate the block descriptors of several cases: case c's blocks get
+    case_id c and their interface neighbour indices move by the case's block offset.
+    Returns (descs, per-case block offsets)."""
+    nblk = sum(len(d) for d in case_descs)
+    descs = (_lib.BlockDesc * nblk)()
+    offsets, k = [], 0
+    for c, cd in enumerate(case_descs):
+        offsets.append(k)
+        for d in cd:
+            nd = _lib.BlockDesc()
+            C.memmove(C.byref(nd), C.byref(d), C.sizeof(_lib.BlockDesc))
+            nd.case_id = c
+            for f in range(6):
+                if nd.faces[f].kind == _lib.FACE_INTERFACE:
+                    nd.faces[f].nb += offsets[c]
+                if nd.faces[f].kind == _lib.BC_INFLOW:
+                    raise ValueError("time-dependent inflow cases are not batched")
+            descs[k] = nd
+            k += 1
+    return descs, offsets
 
 
 class EnsembleSolver:
@@ -23,9 +52,8 @@ class EnsembleSolver:
             raise ValueError("an ensemble holds 1..16 cases")
         s0 = solvers[0]
         for s in solvers[1:]:
-            same = (s.gas == s0.gas and type(s.scheme) is type(s0.scheme) and s.sgs.code == s0.sgs.code
-                    and s.sgs.coef == s0.sgs.coef and s.cfl == s0.cfl and s.precision == s0.precision
-                    and s.bc_passes == s0.bc_passes and not s.sources and not s0.sources)
+            same = (s.gas == s0.gas and _numerics_key(s._nd) == _numerics_key(s0._nd)
+                    and s.precision == s0.precision and not s.sources and not s0.sources)
             if not same:
                 raise ValueError("ensemble cases must share gas, scheme, SGS, CFL, precision and have no sources")
         nblk = sum(len(s.blocks) for s in solvers)
@@ -34,30 +62,18 @@ class EnsembleSolver:
         self.solvers = solvers
         self.precision = s0.precision
         self._dev = s0._dev
-        descs = (_lib.BlockDesc * nblk)()
-        self._offsets, k = [], 0
-        for c, s in enumerate(solvers):
-            self._offsets.append(k)
-            for d in s._descs:
-                nd = _lib.BlockDesc()
-                C.memmove(C.byref(nd), C.byref(d), C.sizeof(_lib.BlockDesc))
-                nd.case_id = c
-                for f in range(6):
-                    if nd.faces[f].kind == _lib.FACE_INTERFACE:
-                        nd.faces[f].nb += k
-                    if nd.faces[f].kind == _lib.BC_INFLOW:
-                        raise ValueError("time-dependent inflow cases are not batched")
-                descs[k] = nd
-                k += 1
+        descs, self._offsets = merge_block_descs([s._descs for s in solvers])
         self._descs = descs
         cons = []
         for s in solvers:
             if len(s._native_constraints) != 1 or s._py_constraints:
                 raise ValueError("each ensemble case needs exactly one ConstantMassFlux constraint")
             cons.append(s._native_constraints[0])
+        self.wall_model = wall_model_of([b for s in solvers for b in s.blocks])
         self._nd = numerics_desc(s0.gas, s0.scheme, s0.sgs, s0.cfl, s0.bc_passes, cons, (),
                                  n_cases=len(solvers),
-                                 precision=_lib.PREC_FP32_BULK if self.precision == "fp32" else _lib.PREC_FP64)
+                                 precision=_lib.PREC_FP32_BULK if self.precision == "fp32" else _lib.PREC_FP64,
+                                 wall_model=self.wall_model)
         plan = C.c_void_p()
         _lib.check(_lib.lib.wmx_plan_create(nblk, descs, C.byref(self._nd), 0, None, C.byref(plan)))
         self._plan = plan
@@ -68,6 +84,9 @@ class EnsembleSolver:
         self._stream = torch.cuda.Stream(device=self._dev)
         self._dt_dev = torch.empty(16, dtype=torch.float64, device=self._dev)
         self._gauge = None
+        from .learned_wm import LearnedWallModel
+        if isinstance(self.wall_model, LearnedWallModel):
+            _lib.check(_lib.lib.wmx_plan_set_wall_net(self._plan, C.byref(self.wall_model.desc())))
 
     def __del__(self):
         plan = getattr(self, "_plan", None)
@@ -147,6 +166,23 @@ class EnsembleSolver:
         for q in flat:
             q.record_stream(self._stream)
         return self._split(out), dts
+
+    KERNEL_CLASSES = ("flux", "grad", "prim", "fill", "wall", "dt", "massflux", "shell")
+
+    def profile(self, classes=("flux",)):
+        """Start HIP-event timing of the named kernel classes (empty: stop).  Launches
+        replayed from a captured graph are not timed."""
+        mask = 0
+        for c in classes:
+            mask |= 1 << self.KERNEL_CLASSES.index(c)
+        _lib.check(_lib.lib.wmx_plan_profile(self._plan, mask))
+
+    def profile_read(self):
+        """{class: (total_ms, launches)} of the launches timed since profile()."""
+        ms = (C.c_double * 8)()
+        cnt = (C.c_int32 * 8)()
+        _lib.check(_lib.lib.wmx_plan_profile_read(self._plan, ms, cnt))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES) if cnt[i]}
 
     def set_tiling(self, ci_len=16, order=0):
         _lib.check(_lib.lib.wmx_plan_set_tiling(self._plan, int(ci_len), int(order)))

@@ -48,8 +48,32 @@ class _Ctx:
         self.inflows = []
 
 
+def wall_model_of(blocks):
+    """The one wall-model object the WallModel faces of `blocks` use (None without
+    wall-modelled faces).  The wall kernel evaluates a single model per plan."""
+    models = []
+    for b in blocks:
+        for bc in b.faces.values():
+            if isinstance(bc, WallModel) and not any(bc.model is m for m in models):
+                models.append(bc.model)
+    if not models:
+        return None
+    from .learned_wm import LearnedWallModel
+    from .wall_model import EquilibriumWallModel
+    for m in models:
+        if not isinstance(m, (EquilibriumWallModel, LearnedWallModel)):
+            raise NotImplementedError(f"wall model {type(m).__name__} has no HIP implementation")
+    first = models[0]
+    for m in models[1:]:
+        same = (type(m) is type(first) and isinstance(m, EquilibriumWallModel)
+                and (m.kappa, m.B, m.n_iter) == (first.kappa, first.B, first.n_iter))
+        if not same:
+            raise NotImplementedError("all WallModel faces of a solver must share one wall model")
+    return first
+
+
 def numerics_desc(gas, scheme, sgs, cfl, bc_passes, constraints, sources, n_cases=1,
-                  precision=_lib.PREC_FP64):
+                  precision=_lib.PREC_FP64, wall_model=None):
     nd = _lib.NumericsDesc()
     nd.gamma, nd.R, nd.Pr, nd.Pr_t = gas.gamma, gas.R, gas.Pr, gas.Pr_t
     nd.mu_ref, nd.T_ref, nd.S_suth = gas.mu_ref, gas.T_ref, gas.S_suth
@@ -71,7 +95,7 @@ def numerics_desc(gas, scheme, sgs, cfl, bc_passes, constraints, sources, n_case
     sg = sgs if isinstance(sgs, SGSModel) else make_sgs(None)
     nd.sgs, nd.sgs_c = sg.code, sg.coef
     from .wall_model import EquilibriumWallModel
-    wm = EquilibriumWallModel()
+    wm = wall_model if wall_model is not None else EquilibriumWallModel()
     nd.kappa, nd.B, nd.n_iter = wm.kappa, wm.B, wm.n_iter
     nd.wm_ekb, nd.wm_pr13, nd.wm_pr3, nd.wm_beta = wm.host_constants(gas)
     nd.bc_passes = int(bc_passes)
@@ -130,8 +154,10 @@ class Solver:
         self._keep = []
         descs = [block_desc(b, mesh.index, self._ctx, g) for b, g in zip(self.blocks, self.geoms)]
         self._descs = (_lib.BlockDesc * len(descs))(*descs)
+        self.wall_model = wall_model_of(self.blocks)
         self._nd = numerics_desc(gas, scheme, self.sgs, cfl, bc_passes, native, self.sources,
-                                 precision=_lib.PREC_FP32_BULK if precision == "fp32" else _lib.PREC_FP64)
+                                 precision=_lib.PREC_FP32_BULK if precision == "fp32" else _lib.PREC_FP64,
+                                 wall_model=self.wall_model)
         inflows = (_lib.InflowDesc * max(1, len(self._ctx.inflows)))()
         for i, inf in enumerate(self._ctx.inflows):
             inflows[i] = inf.desc(self._dev)
@@ -152,6 +178,22 @@ class Solver:
         self._dt_buf = torch.empty(16, dtype=torch.float64, device=self._dev)
         if gauge is not None:
             self.set_gauge(*gauge)
+        from .learned_wm import LearnedWallModel
+        if isinstance(self.wall_model, LearnedWallModel):
+            self.set_wall_net(self.wall_model)
+
+    def set_wall_net(self, model):
+        """Evaluate the wall-modelled faces with a trained LearnedWallModel (None: the
+        Newton model)."""
+        if model is None:
+            _lib.check(_lib.lib.wmx_plan_set_wall_net(self._plan, None))
+            self._wall_net = None
+            return
+        if model.params is None:
+            raise RuntimeError("LearnedWallModel has no parameters: train() or load() it before "
+                               "building the Solver")
+        self._wall_net = model
+        _lib.check(_lib.lib.wmx_plan_set_wall_net(self._plan, C.byref(model.desc())))
 
     def set_gauge(self, p_ref, T_ref):
         """Reference pressure / temperature subtracted in the FP32-bulk records."""
