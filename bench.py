@@ -162,6 +162,8 @@ def bench_channel(args, wmx):
                              "form": head},
                   _roofline(flux_ms / flux_n / 1e3, b_launch, workload))
     rec["step_hbm_frac"] = round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
+    ci, order, ntiles = r["solver"].tiling()
+    rec["config"]["tiling"] = {"ci_len": ci, "xcd_order": order, "tiles": ntiles}
     if "fp64" in res and "fp32" in res:
         v64 = cells * args.steps / res["fp64"]["elapsed"] / 1e6
         rec["fp64_value"] = round(v64, 2)

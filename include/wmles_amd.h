@@ -211,9 +211,17 @@ int wmx_advance(wmx_plan* plan, double* const* q0, double* const* qout, double d
    kept per plan) and replayed on later calls with the same pointers. */
 int wmx_step(wmx_plan* plan, double* const* q0, double* const* qout, double* d_dt, double time,
              int32_t use_graph, void* stream);
-/* Flux-kernel tiling (autotuner knob): ci_len i-planes per marching tile; order 0
-   (block/i/j/k) or 1 (contiguous spans interleaved across the 8 XCDs). */
+/* Flux-kernel tiling (autotuner knob; no counterpart in the reference): ci_len
+   i-planes per marching tile (0: chosen from a makespan model of the device's CU
+   count, the default); order 0 (block/i/j/k) or 1 (contiguous spans interleaved
+   across the 8 XCDs, the default). */
 int wmx_plan_set_tiling(wmx_plan* plan, int32_t ci_len, int32_t order);
+/* The automatic choice without a plan (host only): i-planes per tile and tile count
+   for blocks of interior sizes dims[3 * nblocks] in the given form on cu_count CUs. */
+int wmx_tile_model(int32_t nblocks, const int32_t* dims, int32_t precision, int32_t cu_count,
+                   int32_t* ci_len, int32_t* ntiles);
+/* The tiling in use: i-planes per tile, order, number of tiles. */
+int wmx_plan_tiling(const wmx_plan* plan, int32_t* ci_len, int32_t* order, int32_t* ntiles);
 /* CFL time step (per case when the plan is an ensemble) into d_dt[n_cases].
    Replaces Solver.max_dt / _dt (wmles/solver.py:135-154). */
 int wmx_max_dt(wmx_plan* plan, double* const* q, double* d_dt, void* stream);

@@ -133,3 +133,35 @@ def test_ensemble_descriptor_merge_offsets_neighbours():
     assert [d.faces[1].nb for d in descs] == [1, 0, 3, 4, 2, 5]
     assert [d.n[0] for d in descs] == [10, 11, 10, 11, 12, 10]
     assert all(d.faces[2].nb == 0 for d in descs)             # non-interface faces untouched
+
+
+def test_tile_model_picks_the_shortest_makespan():
+    """Automatic i-planes per tile: ceil(T / 2 CUs) rounds of (ci + alpha) planes."""
+    import ctypes as C
+    from wmles_jax_amd import _lib
+
+    def model(dims, prec, cus=256):
+        arr = (C.c_int32 * len(dims))(*dims)
+        ci, nt = C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib.wmx_tile_model(len(dims) // 3, arr, prec, cus, C.byref(ci), C.byref(nt)))
+        return ci.value, nt.value
+
+    def brute(dims, prec, cus=256):
+        tj, alpha = (15, 1.5) if prec == _lib.PREC_FP32_BULK else (16, 0.5)
+        best = None
+        for ci in range(32, 3, -1):
+            T = sum(-(-dims[3 * b] // ci) * -(-dims[3 * b + 1] // tj) * -(-dims[3 * b + 2] // 16)
+                    for b in range(len(dims) // 3))
+            cost = -(-T // (2 * cus)) * (ci + alpha)
+            if best is None or cost < best[0] - 1e-9:
+                best = (cost, ci, T)
+        return best[1], best[2]
+
+    for dims in ([256, 128, 128], [512, 256, 256], [512, 128, 64], [72, 56, 40, 33, 39, 56],
+                 [100, 30, 17]):
+        for prec in (_lib.PREC_FP64, _lib.PREC_FP32_BULK):
+            assert model(dims, prec) == brute(dims, prec)
+    # the 256 x 128 x 128 channel: 72 tiles of 15 x 16 per i chunk; 14 chunks of 19 planes
+    # make 1008 tiles, two rounds of 512 slots
+    assert model([256, 128, 128], _lib.PREC_FP32_BULK) == (19, 1008)
+    assert model([256, 128, 128], _lib.PREC_FP64)[1] % 512 == 0

@@ -90,6 +90,8 @@ def _load():
         "wmx_max_dt": ([P, PP, P, P], C.c_int),
         "wmx_step": ([P, PP, PP, P, C.c_double, C.c_int32, P], C.c_int),
         "wmx_plan_set_tiling": ([P, C.c_int32, C.c_int32], C.c_int),
+        "wmx_tile_model": ([C.c_int32, P, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "wmx_plan_tiling": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "wmx_mass_flux": ([P, PP, P], C.c_int),
         "wmx_wall_stats": ([P, PP, P, P, P], C.c_int),
         "wmx_fill_face": ([C.POINTER(BlockDesc), C.c_int32, C.c_int32, C.POINTER(NumericsDesc), P,

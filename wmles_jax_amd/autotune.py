@@ -1,7 +1,8 @@
 """Flux-kernel tiling autotuner (project-written).
 
 Times a few steps of a solver (or ensemble) for each candidate tiling -- i-planes per
-marching tile x XCD-interleaved tile order -- with HIP events around the flux kernel,
+marching tile (0: the plan's makespan model) x XCD-interleaved tile order -- with HIP
+events around the flux kernel,
 keeps the fastest, and caches the choice per (device, form, block sizes) in a JSON file
 (default ~/.cache/wmles_jax_amd/tiling.json, override with WMX_TUNE_CACHE).
 """
@@ -10,7 +11,7 @@ import os
 
 import torch
 
-CANDIDATES = ((8, 0), (16, 0), (32, 0), (8, 1), (16, 1), (32, 1))
+CANDIDATES = ((0, 1), (0, 0), (8, 1), (12, 1), (16, 1), (24, 1), (32, 1))
 
 
 def _cache_path():

@@ -80,8 +80,10 @@ struct wmx_plan {
   // flux tiles
   std::vector<int4> tiles;
   int4* d_tiles = nullptr;
-  int ci_len = 16;                    // i-planes per marching tile
-  int tile_order = 0;                 // 0: block/i/j/k order, 1: XCD-interleaved
+  int ci_req = 0;                     // requested i-planes per marching tile (0: auto)
+  int ci_len = 16;                    // i-planes per marching tile in use
+  int tile_order = 1;                 // 0: block/i/j/k order, 1: XCD-interleaved
+  int cu_count = 256;                 // compute units of the device the plan was made on
   // captured steps: (q0, qout, d_dt, time) -> graph of max_dt + advance
   struct GraphEntry {
     std::vector<double*> q0, qout;
@@ -309,12 +311,46 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // Flux tiles: per block (i-chunk, j-tile, k-tile).  Order 1 interleaves contiguous
 // spans of the tile list across the 8 XCDs (workgroup w runs on XCD w % 8), so the
 // tiles one XCD runs are neighbours and share its L2.
+// i-planes per tile when none is requested: both flux kernels run two 256-thread
+// workgroups per CU, so a launch of T tiles takes ceil(T / (2 CUs)) rounds of
+// (ci + alpha) plane times, alpha being the per-tile start-up (the FP32-bulk kernel's
+// register window costs about 1.5 planes, the FP64 kernel's about 0.5).  The shortest
+// estimate over ci in [4, 32] wins (ties: the longer chunk).  dims: n[3] per block.
+int auto_ci_len(int nb, const int* dims, int prec, int cu_count, long long* ntiles) {
+  const int tjr = prec == WMX_PREC_FP32_BULK ? FLUX2_TILE_J : 16;
+  const double alpha = prec == WMX_PREC_FP32_BULK ? 1.5 : 0.5;
+  const long long slots = 2LL * std::max(1, cu_count);
+  int best = 16;
+  long long best_T = 0;
+  double best_cost = 1e300;
+  for (int ci = 32; ci >= 4; --ci) {
+    long long T = 0;
+    for (int b = 0; b < nb; ++b) {
+      const int* n = dims + 3 * b;
+      T += (long long)((n[0] + ci - 1) / ci) * ((n[1] + tjr - 1) / tjr) * ((n[2] + 15) / 16);
+    }
+    const double cost = (double)((T + slots - 1) / slots) * (ci + alpha);
+    if (cost < best_cost - 1e-9) { best_cost = cost; best = ci; best_T = T; }
+  }
+  if (ntiles) *ntiles = best_T;
+  return best;
+}
+
 void build_tiles(wmx_plan* p) {
   std::vector<int4> t;
+  // tile rows: 16 for the FP64 flux kernel, 15 for the FP32-bulk one
+  const int tjr = p->nd.prec == WMX_PREC_FP32_BULK ? FLUX2_TILE_J : 16;
+  if (p->ci_req > 0) {
+    p->ci_len = p->ci_req;
+  } else {
+    std::vector<int> dims;
+    for (int b = 0; b < p->nb; ++b) dims.insert(dims.end(), p->hb[b].n, p->hb[b].n + 3);
+    p->ci_len = auto_ci_len(p->nb, dims.data(), p->nd.prec, p->cu_count, nullptr);
+  }
   for (int b = 0; b < p->nb; ++b) {
     const BlockDev& B = p->hb[b];
     for (int i0 = 0; i0 < B.n[0]; i0 += p->ci_len)
-      for (int j0 = 0; j0 < B.n[1]; j0 += 16)
+      for (int j0 = 0; j0 < B.n[1]; j0 += tjr)
         for (int k0 = 0; k0 < B.n[2]; k0 += 16) t.push_back(make_int4(b, j0, k0, i0));
   }
   if (p->tile_order == 1) {
@@ -531,6 +567,12 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
         p->wall.push_back(WallOp{b, f / 2, f % 2, cnt});
         p->wall_max = std::max(p->wall_max, cnt);
       }
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      p->cu_count = cus;
+  }
   build_tiles(p);
   // prefix tables
   std::vector<long long> c_all, c_box, c_int, c_shell;
@@ -846,11 +888,28 @@ int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
   return 0;
 }
 
+int wmx_tile_model(int32_t nblocks, const int32_t* dims, int32_t precision, int32_t cu_count,
+                   int32_t* ci_len, int32_t* ntiles) {
+  if (nblocks < 1 || !dims || !ci_len || !ntiles) return fail("bad argument");
+  long long T = 0;
+  *ci_len = auto_ci_len(nblocks, dims, precision, cu_count, &T);
+  *ntiles = (int32_t)T;
+  return 0;
+}
+
+int wmx_plan_tiling(const wmx_plan* p, int32_t* ci_len, int32_t* order, int32_t* ntiles) {
+  if (!p || !ci_len || !order || !ntiles) return fail("null argument");
+  *ci_len = p->ci_len;
+  *order = p->tile_order;
+  *ntiles = (int32_t)p->tiles.size();
+  return 0;
+}
+
 int wmx_plan_set_tiling(wmx_plan* p, int32_t ci_len, int32_t order) {
   if (!p) return fail("null plan");
-  if (ci_len < 1 || ci_len > 1024) return fail("ci_len must be in [1, 1024]");
+  if (ci_len < 0 || ci_len > 1024) return fail("ci_len must be in [0 (auto), 1024]");
   if (order != 0 && order != 1) return fail("order must be 0 or 1");
-  p->ci_len = ci_len;
+  p->ci_req = ci_len;
   p->tile_order = order;
   build_tiles(p);
   (void)hipFree(p->d_tiles);

@@ -68,6 +68,8 @@ struct NumDev {
   WallNetDev net;
 };
 
+constexpr int FLUX2_TILE_J = 15;   // cell rows of an FP32-bulk flux tile (flux.hip)
+
 struct GhostOp {
   int blk;       // destination block
   int axis, side;

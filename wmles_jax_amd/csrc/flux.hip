@@ -64,12 +64,60 @@ __device__ __forceinline__ ColGT<R> load_colg(const R* __restrict__ G, long long
 // e_ref m + p_ref u_bar.S (energy) is added back in FP64 with the FP64 face vector.
 // MUSCL-HLLC reconstructs the shifted pressure and adds p_ref before the Riemann
 // solve.  The cell-to-cell unit vector and distance come from FP64 centroids.
+// FP32-bulk face: blend, viscous flux and area products in FP32; the FP64 result is
+// the FP32 flux plus (1 - s) times the central part's gauge contribution.
+__device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
+                                              const CellWT<float>& L, const CellWT<float>& Rc,
+                                              const CellWT<float>& R2, float thL, float thR,
+                                              const CellGT<float>& gL, const CellGT<float>& gR,
+                                              float TL, float TR, const double* xL, const double* xR,
+                                              double Sx, double Sy, double Sz, double* F) {
+  const float sx = float(Sx), sy = float(Sy), sz = float(Sz);
+  const int sch = nd.scheme;
+  float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
+  if (sch == WMX_SCHEME_HYBRID) {
+    const float th = fmaxf(thL, thR);
+    s = fminf(fmaxf(fdiv<float>(th - float(nd.th0), float(nd.th1 - nd.th0)), 0.f), 1.f);
+  }
+  float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Fu[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
+  if (s < 1.f) {
+    const bool keep = sch == WMX_SCHEME_KEEP || (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP);
+    if (keep) central_flux<true, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
+    else central_flux<false, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
+  }
+  if (s > 0.f) muscl_hllc_flux<float>(float(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz,
+                                      float(nd.p_ref), Fu);
+  const float wc = 1.f - s;
+  float Ff[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v] + s * Fu[v];
+  const float d0 = float(xR[0] - xL[0]), d1 = float(xR[1] - xL[1]), d2 = float(xR[2] - xL[2]);
+  const float dist = fsqrt<float>((d0 * d0 + d1 * d1) + d2 * d2);
+  const float id = fdiv<float>(1.f, dist);
+  const float phL[4] = {L.u, L.v, L.w, TL};
+  const float phR[4] = {Rc.u, Rc.v, Rc.w, TR};
+  viscous_face_e<float>(gL, gR, phL, phR, d0 * id, d1 * id, d2 * id, dist, sx, sy, sz, Ff);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) F[v] = (double)Ff[v];
+  if (s < 1.f) {
+    const double w = (double)wc;
+    F[1] += w * (nd.p_ref * Sx);
+    F[2] += w * (nd.p_ref * Sy);
+    F[3] += w * (nd.p_ref * Sz);
+    F[4] += w * (nd.e_ref * (double)Fc[0] + nd.p_ref * (double)un4);
+  }
+}
+
 template <class R>
 __device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2, const CellWT<R>& L,
                                           const CellWT<R>& Rc, const CellWT<R>& R2, R thL, R thR,
                                           const CellGT<R>& gL, const CellGT<R>& gR, R TL, R TR,
                                           const double* xL, const double* xR, double Sx, double Sy,
                                           double Sz, double* F) {
+  if constexpr (sizeof(R) == 4) {
+    flux_core_f32(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, Sx, Sy, Sz, F);
+    return;
+  }
   const R sx = R(Sx), sy = R(Sy), sz = R(Sz);
   const int sch = nd.scheme;
   const bool gauge = sizeof(R) == 4;
@@ -120,10 +168,22 @@ __device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2,
   for (int v = 0; v < 5; ++v) F[v] = F[v] + (double)Fv[v];
 }
 
-__device__ __forceinline__ void ring_slot(int t, int h, int* sj, int* sk) {
+// FP32-bulk tile: 15 rows (j) x 16 columns (k) of cells on a 16 x 16 thread grid.
+// Per plane the tile has 240 i faces, 16 x 16 j faces (incl. the upper edge row) and
+// 15 x 17 k faces (incl. the upper edge column): thread (tj, tk) evaluates the i face
+// of its cell (tj < 15), the j face below row tj, and the k face left of its cell
+// (tj < 15) or the upper edge k face of row tk (tj = 15) -- three face tasks for every
+// wavefront, none waiting at the plane barrier for a fourth.
+constexpr int C2J = FLUX2_TILE_J;              // cell rows of an FP32-bulk tile
+constexpr int H2J = C2J + 4, H2K = TK + 4;     // primitive region (halo 2)
+constexpr int G2J = C2J + 2, G2K = TK + 2;     // gradient / centroid region (halo 1)
+constexpr int NP2 = H2J * H2K, NG2 = G2J * G2K;
+
+// slot t of the halo ring of width h around a rows x TK interior
+__device__ __forceinline__ void ring_slot(int t, int h, int rows, int* sj, int* sk) {
   const int W = TK + 2 * h, top = h * W;
   if (t < top) { *sj = t / W; *sk = t % W; return; }
-  if (t < 2 * top) { t -= top; *sj = TJ + h + t / W; *sk = t % W; return; }
+  if (t < 2 * top) { t -= top; *sj = rows + h + t / W; *sk = t % W; return; }
   t -= 2 * top;
   const int row = t / (2 * h), c = t % (2 * h);
   *sj = h + row;
@@ -131,14 +191,14 @@ __device__ __forceinline__ void ring_slot(int t, int h, int* sj, int* sk) {
 }
 
 template <class R>
-__device__ __forceinline__ CellWT<R> sp_w(const R (*sP)[HJ * HK], int s) {
+__device__ __forceinline__ CellWT<R> sp_w(const R (*sP)[NP2], int s) {
   CellWT<R> w;
   w.r = sP[0][s]; w.u = sP[1][s]; w.v = sP[2][s]; w.w = sP[3][s]; w.p = sP[4][s]; w.e = sP[5][s];
   return w;
 }
 
 template <class R>
-__device__ __forceinline__ CellGT<R> sg_g(const R (*sG)[GJ * GK], int s) {
+__device__ __forceinline__ CellGT<R> sg_g(const R (*sG)[NG2], int s) {
   CellGT<R> g;
 #pragma unroll
   for (int i = 0; i < 12; ++i) g.g[i] = sG[i][s];
@@ -157,11 +217,12 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
                                                const double* __restrict__ d_dt, double dt_host) {
   // staging area: primitives (halo 2) and gradient records (halo 1) in R; the FP64
   // face fluxes of the plane go to their own exchange buffers as they are computed
-  __shared__ R sP[NPW][HJ * HK];
-  __shared__ R sG[NGW][GJ * GK];
-  __shared__ double sX[3][GJ * GK];
-  __shared__ double sFj[5][(TJ + 1) * TK];
-  __shared__ double sFk[5][TJ * (TK + 1)];
+  __shared__ R sP[NPW][NP2];
+  __shared__ R sG[NGW][NG2];
+  __shared__ double sX[3][NG2];
+  __shared__ double sFj[5][(C2J + 1) * TK];
+  __shared__ double sFk[5][C2J * (TK + 1)];
+  __shared__ double sFi[2][5][C2J * TK];        // own i faces: lower / upper, ping-pong
   const int4 t = tiles[blockIdx.x];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
@@ -172,8 +233,9 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   const int* N = B.N;
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
   const int j = j0 + tj, k = k0 + tk;
-  const bool act = j < n1 && k < n2;
-  const int jl = min(j0 + TJ, n1) - j0;
+  const bool own = tj < C2J;                     // thread owns a column of the tile
+  const bool act = own && j < n1 && k < n2;
+  const int jl = min(j0 + C2J, n1) - j0;
   const int kl = min(k0 + TK, n2) - k0;
   const int J = ng + j, K = ng + k;
   const long long cells = B.cells;
@@ -200,7 +262,8 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
 #pragma unroll
     for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I0 - 1) * sI + cb];
   }
-  double Flo[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Fhi[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int own_s = tj * TK + tk;               // this column's i-face slot (own threads)
+  int lo = 0;                                    // slot of the current lower i face
   for (int I = I0 - 1; I < ng + i1; ++I) {
     const bool first = I == I0 - 1;
     const long long cb = (long long)J * N[2] + K;
@@ -218,7 +281,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     }
     // ---- stage plane I in LDS: own columns from registers, halo and idle columns from HBM
     if (!first) {
-      const int sp = (tj + 2) * HK + (tk + 2), sg = (tj + 1) * GK + (tk + 1);
+      const int sp = (tj + 2) * H2K + (tk + 2), sg = (tj + 1) * G2K + (tk + 1);
       const long long pl = (long long)I * sI;
       if (act) {
         sP[0][sp] = Pw[1].w.r; sP[1][sp] = Pw[1].w.u; sP[2][sp] = Pw[1].w.v; sP[3][sp] = Pw[1].w.w;
@@ -228,7 +291,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         sG[12][sg] = Gw[0].g.mu; sG[13][sg] = Gw[0].g.k; sG[14][sg] = Gw[0].th;
 #pragma unroll
         for (int x = 0; x < 3; ++x) sX[x][sg] = Xw[0][x];
-      } else {
+      } else if (own) {
         if (p_ok(J, K)) {
           const long long c = pl + (long long)J * N[2] + K;
           sP[0][sp] = P[c]; sP[1][sp] = P[cells + c]; sP[2][sp] = P[2 * cells + c];
@@ -241,37 +304,40 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           for (int x = 0; x < 3; ++x) sX[x][sg] = XC[x * cells + c];
         }
       }
-      // halo rings: 144 primitive slots (halo 2), 68 gradient/centroid slots (halo 1)
-      if (tid < 144) {
+      // halo rings: 140 primitive slots (halo 2), 66 gradient/centroid slots (halo 1)
+      constexpr int NRP = NP2 - C2J * TK, NRG = NG2 - C2J * TK;
+      if (tid < NRP) {
         int sj, sk;
-        ring_slot(tid, 2, &sj, &sk);
+        ring_slot(tid, 2, C2J, &sj, &sk);
         const int jj = ng + j0 - 2 + sj, kk = ng + k0 - 2 + sk;
         if (p_ok(jj, kk)) {
           const long long c = pl + (long long)jj * N[2] + kk;
-          const int s = sj * HK + sk;
+          const int s = sj * H2K + sk;
           sP[0][s] = P[c]; sP[1][s] = P[cells + c]; sP[2][s] = P[2 * cells + c];
           sP[3][s] = P[3 * cells + c]; sP[4][s] = P[4 * cells + c];
           sP[5][s] = P[P_E * cells + c]; sP[6][s] = P[P_T * cells + c];
         }
-      } else if (tid < 144 + 68) {
+      } else if (tid < NRP + NRG) {
         int sj, sk;
-        ring_slot(tid - 144, 1, &sj, &sk);
+        ring_slot(tid - NRP, 1, C2J, &sj, &sk);
         const int jj = ng + j0 - 1 + sj, kk = ng + k0 - 1 + sk;
         if (g_ok(jj, kk)) {
           const long long c = pl + (long long)jj * N[2] + kk;
-          const int s = sj * GK + sk;
+          const int s = sj * G2K + sk;
           for (int i = 0; i < NGW; ++i) sG[i][s] = G[i * cells + c];
           for (int x = 0; x < 3; ++x) sX[x][s] = XC[x * cells + c];
         }
       }
     }
     __syncthreads();
-    // ---- face fluxes: 0 i face I+1 (registers), 1 j face J, 2 k face K, 3 tile-edge faces
-    const int ntask = first ? 1 : (tid < 64 ? 4 : 3);
+    // ---- face fluxes: 0 i face I+1 (registers), 1 j face below row tj, 2 k face left of
+    //      column tk (rows tj < 15) or the upper edge k face of row tk (tj = 15)
+    const int ntask = first ? 1 : 3;
 #pragma unroll 1
     for (int task = 0; task < ntask; ++task) {
       bool doit = act;
       int d = 0, fI = I, fJ = J, fK = K;
+      int rj = tj, rk = tk;                          // tile coords of the face's right cell
       CellWT<R> L2, L, Rc, R2;
       CellGT<R> gL, gR;
       R thL, thR, TL, TR;
@@ -284,17 +350,22 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
 #pragma unroll
         for (int x = 0; x < 3; ++x) { xL[x] = Xw[0][x]; xR[x] = Xw[1][x]; }
       } else {
-        int rj = tj, rk = tk;                        // tile coords of the face's right cell
-        if (task == 1) { d = 1; }
-        else if (task == 2) { d = 2; }
-        else if (tid < TK) { d = 1; rj = jl; rk = tid; doit = tid < kl; }
-        else if (tid < TK + TJ) { d = 2; rj = tid - TK; rk = kl; doit = (tid - TK) < jl; }
-        else doit = false;
+        if (task == 1) {
+          d = 1;
+          doit = rj <= jl && rk < kl;
+        } else if (own) {
+          d = 2;
+          doit = rj < jl && rk < kl;
+        } else {
+          d = 2;
+          rj = tk; rk = kl;
+          doit = rj < jl;
+        }
         fJ = ng + j0 + rj;
         fK = ng + k0 + rk;
         const int dj = d == 1, dk = d == 2;
-        const int pR = (rj + 2) * HK + (rk + 2), gR_ = (rj + 1) * GK + (rk + 1);
-        const int pst = dj * HK + dk, gst = dj * GK + dk;
+        const int pR = (rj + 2) * H2K + (rk + 2), gR_ = (rj + 1) * G2K + (rk + 1);
+        const int pst = dj * H2K + dk, gst = dj * G2K + dk;
         if (doit) {
           L2 = sp_w<R>(sP, pR - 2 * pst); L = sp_w<R>(sP, pR - pst); Rc = sp_w<R>(sP, pR);
           R2 = sp_w<R>(sP, pR + pst);
@@ -326,38 +397,31 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           flux_core<R>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, S[fi], S[fc + fi],
                        S[2 * fc + fi], F);
         }
-      }
-      if (doit) {
         if (task == 0) {
+          const int hs = first ? 0 : lo ^ 1;
 #pragma unroll
-          for (int v = 0; v < 5; ++v) Fhi[v] = F[v];
+          for (int v = 0; v < 5; ++v) sFi[hs][v][own_s] = F[v];
+        } else if (d == 1) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sFj[v][rj * TK + rk] = F[v];
         } else {
-          double* dst;
-          int o;
-          if (d == 1) { dst = &sFj[0][0]; o = (fJ - ng - j0) * TK + (fK - ng - k0); }
-          else { dst = &sFk[0][0]; o = (fJ - ng - j0) * (TK + 1) + (fK - ng - k0); }
-          const int stride = d == 1 ? (TJ + 1) * TK : TJ * (TK + 1);
 #pragma unroll
-          for (int v = 0; v < 5; ++v) dst[v * stride + o] = F[v];
+          for (int v = 0; v < 5; ++v) sFk[v][rj * (TK + 1) + rk] = F[v];
         }
       }
     }
-    if (first) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) Flo[v] = Fhi[v];
-      continue;                                      // uniform: no barrier skipped
-    }
+    if (first) continue;                           // uniform: no barrier skipped
     __syncthreads();
     if (act) {
       const long long c = (long long)I * sI + (long long)J * N[2] + K;
-      const double vol = B.vol[c];
+      const double ivol = 1.0 / B.vol[c];          // (FP32-bulk form: one reciprocal)
       double r[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        double a = 0.0 - (Fhi[v] - Flo[v]);
+        double a = 0.0 - (sFi[lo ^ 1][v][own_s] - sFi[lo][v][own_s]);
         a = a - (sFj[v][(tj + 1) * TK + tk] - sFj[v][tj * TK + tk]);
         a = a - (sFk[v][tj * (TK + 1) + tk + 1] - sFk[v][tj * (TK + 1) + tk]);
-        r[v] = a / vol;
+        r[v] = a * ivol;
       }
       const double* qi = qin.p[t.x];
       if (nd.n_sponge > 0) {
@@ -392,8 +456,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           o[v * cells + c] = a0 * qz[v * cells + c] + a1 * (qi[v * cells + c] + dt * r[v]);
       }
     }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) Flo[v] = Fhi[v];
+    lo ^= 1;
     __syncthreads();
   }
 }

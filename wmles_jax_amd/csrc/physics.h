@@ -104,6 +104,25 @@ WMX_HD void bc_farfield(const GasC& g, const double* src, const double* pinf, do
 // is the FP32-bulk form, in which p and e_int arrive gauge-shifted (p - p_ref,
 // e_int - e_ref) and the caller adds the shift's contribution back in FP64.
 
+// Division and square root of the templated bulk arithmetic.  R = double: the IEEE
+// operations (the plain form's parity depends on them).  R = float (device code): the
+// hardware reciprocal / square root (v_rcp_f32, v_sqrt_f32, about 1 ulp), which is
+// well inside the FP32-bulk form's error budget.
+template <class R>
+WMX_HD R fdiv(R a, R b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == 4) return a * __builtin_amdgcn_rcpf(b);
+#endif
+  return a / b;
+}
+template <class R>
+WMX_HD R fsqrt(R a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_sqrtf(a);
+#endif
+  return sqrt(a);
+}
+
 // Primitive record of one cell as the face kernels read it: rho, u, v, w, p, e_int.
 template <class R>
 struct CellWT {
@@ -187,7 +206,7 @@ WMX_HD void central_flux(int order, const CellWT<R>& L2, const CellWT<R>& L, con
 template <class R>
 WMX_HD R albada(R a, R b) {
   const R eps = R(1e-12);
-  return (a * (b * b + eps) + b * (a * a + eps)) / ((a * a + b * b) + R(2.0) * eps);
+  return fdiv<R>(a * (b * b + eps) + b * (a * a + eps), (a * a + b * b) + R(2.0) * eps);
 }
 template <class R>
 WMX_HD R sgn(R a) { return a > R(0) ? R(1) : (a < R(0) ? R(-1) : R(0)); }
@@ -200,18 +219,18 @@ WMX_HD R minmod(R a, R b) {
 // wl, wr: [rho, u, v, w, p] with the full pressure.
 template <class R>
 WMX_HD void hllc_flux(R gamma, const R* wl, const R* wr, R Sx, R Sy, R Sz, R* F) {
-  R Sm = sqrt((Sx * Sx + Sy * Sy) + Sz * Sz);
-  R nx = Sx / Sm, ny = Sy / Sm, nz = Sz / Sm;
+  R Sm = fsqrt<R>((Sx * Sx + Sy * Sy) + Sz * Sz);
+  R nx = fdiv<R>(Sx, Sm), ny = fdiv<R>(Sy, Sm), nz = fdiv<R>(Sz, Sm);
   R rl = wl[0], ul = wl[1], vl = wl[2], zl = wl[3], pl = wl[4];
   R rr = wr[0], ur = wr[1], vr = wr[2], zr = wr[3], pr = wr[4];
   R unl = dot3r(ul, vl, zl, nx, ny, nz), unr = dot3r(ur, vr, zr, nx, ny, nz);
-  R cl = sqrt(gamma * pl / rl), cr = sqrt(gamma * pr / rr);
+  R cl = fsqrt<R>(fdiv<R>(gamma * pl, rl)), cr = fsqrt<R>(fdiv<R>(gamma * pr, rr));
   R sL = fmin(unl - cl, unr - cr), sR = fmax(unl + cl, unr + cr);
-  R ss = (((pr - pl) + (rl * unl) * (sL - unl)) - (rr * unr) * (sR - unr)) /
-         (rl * (sL - unl) - rr * (sR - unr));
+  R ss = fdiv<R>(((pr - pl) + (rl * unl) * (sL - unl)) - (rr * unr) * (sR - unr),
+                 rl * (sL - unl) - rr * (sR - unr));
   const R gm1 = gamma - R(1.0), h = R(0.5);
-  R El = pl / gm1 + (h * rl) * ((ul * ul + vl * vl) + zl * zl);
-  R Er = pr / gm1 + (h * rr) * ((ur * ur + vr * vr) + zr * zr);
+  R El = fdiv<R>(pl, gm1) + (h * rl) * ((ul * ul + vl * vl) + zl * zl);
+  R Er = fdiv<R>(pr, gm1) + (h * rr) * ((ur * ur + vr * vr) + zr * zr);
   R r, u, v, z, p, un, E, s;
   int branch;  // 0: FL, 1: F*L, 2: F*R, 3: FR
   if (sL >= R(0)) branch = 0;
@@ -223,8 +242,8 @@ WMX_HD void hllc_flux(R gamma, const R* wl, const R* wr, R Sx, R Sy, R Sz, R* F)
   R m = r * un;
   R Fp[5] = {m, m * u + p * nx, m * v + p * ny, m * z + p * nz, (E + p) * un};
   if (branch == 1 || branch == 2) {
-    R f = r * (s - un) / (s - ss);
-    R e = E / r + (ss - un) * (ss + p / (r * (s - un)));
+    R f = fdiv<R>(r * (s - un), s - ss);
+    R e = fdiv<R>(E, r) + (ss - un) * (ss + fdiv<R>(p, r * (s - un)));
     R d = ss - un;
     R st[5] = {f, f * (u + d * nx), f * (v + d * ny), f * (z + d * nz), f * e};
     R U[5] = {r, r * u, r * v, r * z, E};
@@ -283,7 +302,7 @@ WMX_HD void viscous_face_e(const CellGT<R>& a, const CellGT<R>& b, const R* phiL
     R g0 = h * (a.g[3 * v + 0] + b.g[3 * v + 0]);
     R g1 = h * (a.g[3 * v + 1] + b.g[3 * v + 1]);
     R g2 = h * (a.g[3 * v + 2] + b.g[3 * v + 2]);
-    R corr = (phiR[v] - phiL[v]) / dist - ((g0 * e0 + g1 * e1) + g2 * e2);
+    R corr = fdiv<R>(phiR[v] - phiL[v], dist) - ((g0 * e0 + g1 * e1) + g2 * e2);
     gf[3 * v + 0] = g0 + corr * e0;
     gf[3 * v + 1] = g1 + corr * e1;
     gf[3 * v + 2] = g2 + corr * e2;
@@ -341,7 +360,7 @@ WMX_HD R nut_smagorinsky(const R* g, R delta, R Cs) {
       ss = ss + s * s;
     }
   R cd = Cs * delta;
-  return (cd * cd) * sqrt(R(2.0) * ss);
+  return (cd * cd) * fsqrt<R>(R(2.0) * ss);
 }
 
 // reference sgs.py:21-30
@@ -363,13 +382,13 @@ WMX_HD R nut_wale(const R* g, R delta, R Cw) {
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      R sd = R(0.5) * (g2[3 * i + j] + g2[3 * j + i]) - (i == j ? tr / R(3.0) : R(0.0));
+      R sd = R(0.5) * (g2[3 * i + j] + g2[3 * j + i]) - (i == j ? fdiv<R>(tr, R(3.0)) : R(0.0));
       sdsd = sdsd + sd * sd;
     }
   R cd = Cw * delta;
-  R num = (cd * cd) * (sdsd * sqrt(sdsd));
-  R den = ((ss * ss) * sqrt(ss) + sqrt(sqrt(sdsd)) * sdsd) + R(1e-30);
-  return num / den;
+  R num = (cd * cd) * (sdsd * fsqrt<R>(sdsd));
+  R den = ((ss * ss) * fsqrt<R>(ss) + fsqrt<R>(fsqrt<R>(sdsd)) * sdsd) + R(1e-30);
+  return fdiv<R>(num, den);
 }
 
 // reference sgs.py:33-42
@@ -388,7 +407,7 @@ WMX_HD R nut_vreman(const R* g, R delta, R Cs) {
       aa = aa + g[3 * i + j] * g[3 * i + j];
     }
   R B = ((((b[0] * b[4] - b[1] * b[1]) + b[0] * b[8]) - b[2] * b[2]) + b[4] * b[8]) - b[5] * b[5];
-  return c * sqrt(fmax(B, R(0.0)) / (aa + R(1e-30)));
+  return c * fsqrt<R>(fdiv<R>(fmax(B, R(0.0)), aa + R(1e-30)));
 }
 
 // reference viscous.py:22-25
@@ -398,7 +417,7 @@ WMX_HD R ducros(const R* g) {
   R a = g[7] - g[5], b = g[2] - g[6], c = g[3] - g[1];
   R w2 = (a * a + b * b) + c * c;
   R d2 = div * div;
-  return d2 / ((d2 + w2) + R(1e-30));
+  return fdiv<R>(d2, (d2 + w2) + R(1e-30));
 }
 
 // ------------------------------------------------------------------------ wall model

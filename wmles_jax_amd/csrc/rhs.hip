@@ -119,12 +119,15 @@ __global__ __launch_bounds__(256) void k_grad(const BlockDev* __restrict__ blks,
   const R volr = R(vol);
   R g[12];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) g[i] = acc[i] / volr;
+  for (int i = 0; i < 12; ++i) g[i] = fdiv<R>(acc[i], volr);
   const R rho = P[c];
   const R mu = P[P_MU * cells + c];
   R mut = R(0.0);
   if (nd.sgs != WMX_SGS_NONE) {
-    const R delta = R(pow(vol, 1.0 / 3.0));
+    // filter width vol^(1/3) (solver.py:82); FP32-bulk: single-precision cube root
+    R delta;
+    if constexpr (sizeof(R) == 4) delta = cbrtf((float)vol);
+    else delta = R(pow(vol, 1.0 / 3.0));
     const R cs = R(nd.sgs_c);
     R nut;
     if (nd.sgs == WMX_SGS_SMAGORINSKY) nut = nut_smagorinsky<R>(g, delta, cs);
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(256) void k_grad(const BlockDev* __restrict__ blks,
   for (int i = 0; i < 12; ++i) G[i * cells + c] = g[i];
   const GasC& gs = nd.gas;
   G[G_MU * cells + c] = mu + mut;
-  G[G_K * cells + c] = R(gs.cp) * (mu / R(gs.Pr) + mut / R(gs.Pr_t));
+  G[G_K * cells + c] = R(gs.cp) * (fdiv<R>(mu, R(gs.Pr)) + fdiv<R>(mut, R(gs.Pr_t)));
   G[G_TH * cells + c] = ducros<R>(g);
 }
 

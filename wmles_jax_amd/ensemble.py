@@ -184,5 +184,13 @@ class EnsembleSolver:
         _lib.check(_lib.lib.wmx_plan_profile_read(self._plan, ms, cnt))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES) if cnt[i]}
 
-    def set_tiling(self, ci_len=16, order=0):
+    def set_tiling(self, ci_len=0, order=1):
+        """Flux-kernel tiling: i-planes per marching tile (0: automatic), XCD-interleaved
+        order (1) or not (0)."""
         _lib.check(_lib.lib.wmx_plan_set_tiling(self._plan, int(ci_len), int(order)))
+
+    def tiling(self):
+        """(i-planes per tile, order, number of tiles) in use."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib.wmx_plan_tiling(self._plan, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value

@@ -331,9 +331,16 @@ class Solver:
             out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
         return out, dt
 
-    def set_tiling(self, ci_len=16, order=0):
-        """Flux-kernel tiling: i-planes per marching tile, XCD-interleaved order (1) or not."""
+    def set_tiling(self, ci_len=0, order=1):
+        """Flux-kernel tiling: i-planes per marching tile (0: automatic), XCD-interleaved
+        order (1) or not (0)."""
         _lib.check(_lib.lib.wmx_plan_set_tiling(self._plan, int(ci_len), int(order)))
+
+    def tiling(self):
+        """(i-planes per tile, order, number of tiles) in use."""
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib.wmx_plan_tiling(self._plan, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     # ------------------------------------------------------------------ diagnostics
     def wall_stats(self, Q):
