@@ -328,13 +328,31 @@ def bench_wm(args, wmx):
     return rec
 
 
+def bench_copy(args, wmx):
+    """Plain device copy of 512 MiB (64 Mi doubles): the known-size kernel the HBM
+    traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) are calibrated against."""
+    import torch
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    el, _ = _timed_steps(lambda _: y.copy_(x), None, args.steps, args.warmup)
+    gbs = 2 * n * 8 * args.steps / el / 1e9
+    return {"metric": "device copy GB/s (counter calibration)", "value": round(gbs, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(el * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "copy 512 MiB", "bytes_read_per_launch": n * 8,
+                       "bytes_written_per_launch": n * 8}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--form", default="both", choices=["both", "fp64", "fp32"])
-    ap.add_argument("--workload", default="channel256", choices=["channel256", "ramp512", "ensemble16", "wm512"])
+    ap.add_argument("--workload", default="channel256",
+                    choices=["channel256", "ramp512", "ensemble16", "wm512", "copy"])
     ap.add_argument("--size", type=int, nargs=3, default=(256, 128, 128))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="time every kernel class (extra events)")
@@ -344,13 +362,14 @@ def main():
               "(no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = {"channel256": 200, "ramp512": 200, "ensemble16": 100, "wm512": 20}[args.workload]
+        args.steps = {"channel256": 200, "ramp512": 200, "ensemble16": 100, "wm512": 20,
+                      "copy": 20}[args.workload]
 
     import torch
     import wmles_jax_amd as wmx
     torch.cuda.set_device(0)
     fn = {"channel256": bench_channel, "ramp512": bench_ramp, "ensemble16": bench_ensemble,
-          "wm512": bench_wm}[args.workload]
+          "wm512": bench_wm, "copy": bench_copy}[args.workload]
     rec = fn(args, wmx)
     print(json.dumps(rec), flush=True)
 

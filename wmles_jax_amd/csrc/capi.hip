@@ -96,6 +96,7 @@ struct wmx_plan {
   // per-kernel workgroup prefix tables
   std::vector<int> pre_all, pre_box, pre_int, pre_shell;
   int *d_pre_all = nullptr, *d_pre_box = nullptr, *d_pre_int = nullptr, *d_pre_shell = nullptr;
+
   // case tables over the interior-cell workgroups
   std::vector<int> case_wg, case_off;
   int *d_case_wg = nullptr, *d_case_off = nullptr, *d_case_dir = nullptr;

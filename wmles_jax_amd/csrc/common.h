@@ -141,6 +141,20 @@ int mlp_nf32();
 hipError_t launch_sponge_mean(const BlockDev* d_blks, int blk, long long rows, Ptrs q,
                               hipStream_t s);
 
+
+// Global-memory views of device pointers.  Loads and stores through them compile to
+// global_load / global_store instead of flat ones; a flat access may alias LDS, so the
+// compiler would otherwise complete each one before the next LDS access (a full memory
+// round trip per element in the LDS-staging loops).
+template <class T>
+using gconst = const __attribute__((address_space(1))) T*;
+template <class T>
+using gmut = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gconst<T> gmem(const T* p) { return (gconst<T>)p; }
+template <class T>
+__device__ __forceinline__ gmut<T> gmem_w(T* p) { return (gmut<T>)p; }
+
 // tangential axes of a face normal to `axis`, increasing order
 WMX_HD void tangents(int axis, int* t1, int* t2) {
   *t1 = axis == 0 ? 1 : 0;

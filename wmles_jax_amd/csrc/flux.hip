@@ -34,8 +34,8 @@ struct ColGT {
   R th;
 };
 
-template <class R>
-__device__ __forceinline__ ColT<R> load_col(const R* __restrict__ P, long long cells, long long c) {
+template <class R, class PT>
+__device__ __forceinline__ ColT<R> load_col(PT P, long long cells, long long c) {
   ColT<R> o;
   o.w.r = P[c];
   o.w.u = P[cells + c];
@@ -47,8 +47,8 @@ __device__ __forceinline__ ColT<R> load_col(const R* __restrict__ P, long long c
   return o;
 }
 
-template <class R>
-__device__ __forceinline__ ColGT<R> load_colg(const R* __restrict__ G, long long cells, long long c) {
+template <class R, class PT>
+__device__ __forceinline__ ColGT<R> load_colg(PT G, long long cells, long long c) {
   ColGT<R> o;
 #pragma unroll
   for (int i = 0; i < 12; ++i) o.g.g[i] = G[i * cells + c];
@@ -66,6 +66,11 @@ __device__ __forceinline__ ColGT<R> load_colg(const R* __restrict__ G, long long
 // solve.  The cell-to-cell unit vector and distance come from FP64 centroids.
 // FP32-bulk face: blend, viscous flux and area products in FP32; the FP64 result is
 // the FP32 flux plus (1 - s) times the central part's gauge contribution.
+__device__ __forceinline__ float hybrid_weight(const NumDev& nd, float thL, float thR) {
+  const float th = fmaxf(thL, thR);
+  return fminf(fmaxf(fdiv<float>(th - float(nd.th0), float(nd.th1 - nd.th0)), 0.f), 1.f);
+}
+
 __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
                                               const CellWT<float>& L, const CellWT<float>& Rc,
                                               const CellWT<float>& R2, float thL, float thR,
@@ -75,22 +80,23 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
   const float sx = float(Sx), sy = float(Sy), sz = float(Sz);
   const int sch = nd.scheme;
   float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
-  if (sch == WMX_SCHEME_HYBRID) {
-    const float th = fmaxf(thL, thR);
-    s = fminf(fmaxf(fdiv<float>(th - float(nd.th0), float(nd.th1 - nd.th0)), 0.f), 1.f);
-  }
+  if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
   float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Fu[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
   if (s < 1.f) {
     const bool keep = sch == WMX_SCHEME_KEEP || (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP);
     if (keep) central_flux<true, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
     else central_flux<false, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
   }
-  if (s > 0.f) muscl_hllc_flux<float>(float(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz,
-                                      float(nd.p_ref), Fu);
+  if (s > 0.f) {
+    muscl_hllc_flux<float>(float(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz,
+                           float(nd.p_ref), Fu);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Fu[v] = s * Fu[v];
+  }
   const float wc = 1.f - s;
   float Ff[5];
 #pragma unroll
-  for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v] + s * Fu[v];
+  for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v] + Fu[v];
   const float d0 = float(xR[0] - xL[0]), d1 = float(xR[1] - xL[1]), d2 = float(xR[2] - xL[2]);
   const float dist = fsqrt<float>((d0 * d0 + d1 * d1) + d2 * d2);
   const float id = fdiv<float>(1.f, dist);
@@ -241,45 +247,38 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   const long long cells = B.cells;
   const long long sI = (long long)N[1] * N[2];
   const double dt = d_dt ? d_dt[B.case_id] : dt_host;
-  const R* __restrict__ P = sizeof(R) == 8 ? (const R*)B.P : (const R*)B.Pf;
-  const R* __restrict__ G = sizeof(R) == 8 ? (const R*)B.G : (const R*)B.Gf;
-  const double* __restrict__ XC = B.xc;
+  const gconst<R> P = gmem(sizeof(R) == 8 ? (const R*)B.P : (const R*)B.Pf);
+  const gconst<R> G = gmem(sizeof(R) == 8 ? (const R*)B.G : (const R*)B.Gf);
+  const gconst<double> XC = gmem(B.xc);
   // boxes the staged data may come from: primitives [0, N), gradients [ng-1, ng+n]
   auto p_ok = [&](int jj, int kk) { return jj >= 0 && jj < N[1] && kk >= 0 && kk < N[2]; };
   auto g_ok = [&](int jj, int kk) {
     return jj >= ng - 1 && jj <= ng + n1 && kk >= ng - 1 && kk <= ng + n2;
   };
   // i-direction register window of the own column
+  // The window is loaded by every thread (idle ones from a clamped, valid column), so
+  // each load lands in its window register directly and stays in flight until used.
   ColT<R> Pw[4];
   ColGT<R> Gw[2];
   double Xw[2][3];
   const int I0 = ng + i0;
-  if (act) {
-    const long long cb = (long long)J * N[2] + K;
+  const long long cb = (long long)min(J, N[1] - 1) * N[2] + min(K, N[2] - 1);
 #pragma unroll
-    for (int s = 1; s < 4; ++s) Pw[s] = load_col<R>(P, cells, (I0 - 3 + s) * sI + cb);
-    Gw[1] = load_colg<R>(G, cells, (I0 - 1) * sI + cb);
+  for (int s = 1; s < 4; ++s) Pw[s] = load_col<R>(P, cells, (I0 - 3 + s) * sI + cb);
+  Gw[1] = load_colg<R>(G, cells, (I0 - 1) * sI + cb);
 #pragma unroll
-    for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I0 - 1) * sI + cb];
-  }
+  for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I0 - 1) * sI + cb];
   const int own_s = tj * TK + tk;               // this column's i-face slot (own threads)
   int lo = 0;                                    // slot of the current lower i face
   for (int I = I0 - 1; I < ng + i1; ++I) {
     const bool first = I == I0 - 1;
-    const long long cb = (long long)J * N[2] + K;
-    // ---- advance the window: load plane I+2 (primitives), I+1 (gradients, centroids)
-    if (act) {
-      Pw[0] = Pw[1]; Pw[1] = Pw[2]; Pw[2] = Pw[3];
-      Pw[3] = load_col<R>(P, cells, (I + 2) * sI + cb);
-      Gw[0] = Gw[1];
-      Gw[1] = load_colg<R>(G, cells, (I + 1) * sI + cb);
+    // ---- shift the window (planes I-1..I+1 of primitives, I of gradients/centroids)
+    Pw[0] = Pw[1]; Pw[1] = Pw[2]; Pw[2] = Pw[3];
+    Gw[0] = Gw[1];
 #pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        Xw[0][x] = Xw[1][x];
-        Xw[1][x] = XC[x * cells + (I + 1) * sI + cb];
-      }
-    }
-    // ---- stage plane I in LDS: own columns from registers, halo and idle columns from HBM
+    for (int x = 0; x < 3; ++x) Xw[0][x] = Xw[1][x];
+    // ---- stage plane I in LDS: own columns from registers, halo and idle columns from
+    //      HBM (each group of loads is issued as a batch before its LDS stores)
     if (!first) {
       const int sp = (tj + 2) * H2K + (tk + 2), sg = (tj + 1) * G2K + (tk + 1);
       const long long pl = (long long)I * sI;
@@ -291,17 +290,26 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         sG[12][sg] = Gw[0].g.mu; sG[13][sg] = Gw[0].g.k; sG[14][sg] = Gw[0].th;
 #pragma unroll
         for (int x = 0; x < 3; ++x) sX[x][sg] = Xw[0][x];
-      } else if (own) {
+      } else if (own) {                              // idle column of a partial tile
+        const long long c = pl + (long long)J * N[2] + K;
         if (p_ok(J, K)) {
-          const long long c = pl + (long long)J * N[2] + K;
-          sP[0][sp] = P[c]; sP[1][sp] = P[cells + c]; sP[2][sp] = P[2 * cells + c];
-          sP[3][sp] = P[3 * cells + c]; sP[4][sp] = P[4 * cells + c];
-          sP[5][sp] = P[P_E * cells + c]; sP[6][sp] = P[P_T * cells + c];
+          R vp[NPW];
+#pragma unroll
+          for (int i = 0; i < NPW; ++i) vp[i] = P[(i < 5 ? i : (i == 5 ? P_E : P_T)) * cells + c];
+#pragma unroll
+          for (int i = 0; i < NPW; ++i) sP[i][sp] = vp[i];
         }
         if (g_ok(J, K)) {
-          const long long c = pl + (long long)J * N[2] + K;
-          for (int i = 0; i < NGW; ++i) sG[i][sg] = G[i * cells + c];
-          for (int x = 0; x < 3; ++x) sX[x][sg] = XC[x * cells + c];
+          R vg[NGW];
+          double vx[3];
+#pragma unroll
+          for (int i = 0; i < NGW; ++i) vg[i] = G[i * cells + c];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) vx[x] = XC[x * cells + c];
+#pragma unroll
+          for (int i = 0; i < NGW; ++i) sG[i][sg] = vg[i];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) sX[x][sg] = vx[x];
         }
       }
       // halo rings: 140 primitive slots (halo 2), 66 gradient/centroid slots (halo 1)
@@ -313,9 +321,11 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         if (p_ok(jj, kk)) {
           const long long c = pl + (long long)jj * N[2] + kk;
           const int s = sj * H2K + sk;
-          sP[0][s] = P[c]; sP[1][s] = P[cells + c]; sP[2][s] = P[2 * cells + c];
-          sP[3][s] = P[3 * cells + c]; sP[4][s] = P[4 * cells + c];
-          sP[5][s] = P[P_E * cells + c]; sP[6][s] = P[P_T * cells + c];
+          R vp[NPW];
+#pragma unroll
+          for (int i = 0; i < NPW; ++i) vp[i] = P[(i < 5 ? i : (i == 5 ? P_E : P_T)) * cells + c];
+#pragma unroll
+          for (int i = 0; i < NPW; ++i) sP[i][s] = vp[i];
         }
       } else if (tid < NRP + NRG) {
         int sj, sk;
@@ -324,17 +334,31 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         if (g_ok(jj, kk)) {
           const long long c = pl + (long long)jj * N[2] + kk;
           const int s = sj * G2K + sk;
-          for (int i = 0; i < NGW; ++i) sG[i][s] = G[i * cells + c];
-          for (int x = 0; x < 3; ++x) sX[x][s] = XC[x * cells + c];
+          R vg[NGW];
+          double vx[3];
+#pragma unroll
+          for (int i = 0; i < NGW; ++i) vg[i] = G[i * cells + c];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) vx[x] = XC[x * cells + c];
+#pragma unroll
+          for (int i = 0; i < NGW; ++i) sG[i][s] = vg[i];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) sX[x][s] = vx[x];
         }
       }
     }
+    // ---- load plane I+2 (primitives) and I+1 (gradients, centroids) into the window;
+    //      issued after the halo loads (completion counts in issue order), needed by
+    //      the last face task
+    Pw[3] = load_col<R>(P, cells, (I + 2) * sI + cb);
+    Gw[1] = load_colg<R>(G, cells, (I + 1) * sI + cb);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I + 1) * sI + cb];
     __syncthreads();
-    // ---- face fluxes: 0 i face I+1 (registers), 1 j face below row tj, 2 k face left of
-    //      column tk (rows tj < 15) or the upper edge k face of row tk (tj = 15)
-    const int ntask = first ? 1 : 3;
+    // ---- face fluxes: 0 j face below row tj, 1 k face left of column tk (rows tj < 15)
+    //      or the upper edge k face of row tk (tj = 15), 2 i face I+1 (registers)
 #pragma unroll 1
-    for (int task = 0; task < ntask; ++task) {
+    for (int task = first ? 2 : 0; task < 3; ++task) {
       bool doit = act;
       int d = 0, fI = I, fJ = J, fK = K;
       int rj = tj, rk = tk;                          // tile coords of the face's right cell
@@ -342,7 +366,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
       CellGT<R> gL, gR;
       R thL, thR, TL, TR;
       double xL[3], xR[3];
-      if (task == 0) {
+      if (task == 2) {
         fI = I + 1;
         L2 = Pw[0].w; L = Pw[1].w; Rc = Pw[2].w; R2 = Pw[3].w;
         TL = Pw[1].T; TR = Pw[2].T;
@@ -350,7 +374,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
 #pragma unroll
         for (int x = 0; x < 3; ++x) { xL[x] = Xw[0][x]; xR[x] = Xw[1][x]; }
       } else {
-        if (task == 1) {
+        if (task == 0) {
           d = 1;
           doit = rj <= jl && rk < kl;
         } else if (own) {
@@ -388,16 +412,16 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           const int cc[3] = {fI, fJ, fK};
           const long long nw = (long long)B.n[t1] * B.n[t2];
           const long long w = (long long)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
-          const double* W = B.wall[2 * d + side];
+          const gconst<double> W = gmem(B.wall[2 * d + side]);
 #pragma unroll
           for (int v = 0; v < 5; ++v) F[v] = W[v * nw + w];
         } else {
           const long long fi = fidx(N, d, fI, fJ, fK), fc = fcount(N, d);
-          const double* S = B.S[d];
+          const gconst<double> S = gmem(B.S[d]);
           flux_core<R>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, S[fi], S[fc + fi],
                        S[2 * fc + fi], F);
         }
-        if (task == 0) {
+        if (task == 2) {
           const int hs = first ? 0 : lo ^ 1;
 #pragma unroll
           for (int v = 0; v < 5; ++v) sFi[hs][v][own_s] = F[v];
@@ -411,10 +435,23 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
       }
     }
     if (first) continue;                           // uniform: no barrier skipped
+    // the epilogue's state and volume loads go out before the barrier
+    const long long c = (long long)I * sI + (long long)J * N[2] + K;
+    const gconst<double> qi = gmem(qin.p[t.x]);
+    double vq[5], vz[5], ivol = 0.0;
+    if (act) {
+      ivol = gmem(B.vol)[c];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) vq[v] = qi[v * cells + c];
+      if (MODE >= 2) {
+        const gconst<double> qz = gmem(q0.p[t.x]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) vz[v] = qz[v * cells + c];
+      }
+    }
     __syncthreads();
     if (act) {
-      const long long c = (long long)I * sI + (long long)J * N[2] + K;
-      const double ivol = 1.0 / B.vol[c];          // (FP32-bulk form: one reciprocal)
+      ivol = 1.0 / ivol;                           // (FP32-bulk form: one reciprocal)
       double r[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -423,7 +460,6 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         a = a - (sFk[v][tj * (TK + 1) + tk + 1] - sFk[v][tj * (TK + 1) + tk]);
         r[v] = a * ivol;
       }
-      const double* qi = qin.p[t.x];
       if (nd.n_sponge > 0) {
         const double xcv = XC[c];
         const int i = I - ng;
@@ -437,23 +473,21 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           const double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
 #pragma unroll
           for (int v = 0; v < 5; ++v)
-            r[v] = r[v] + (-sigma) * (qi[v * cells + c] - B.sponge_mean[v * mplane + mrow]);
+            r[v] = r[v] + (-sigma) * (vq[v] - gmem(B.sponge_mean)[v * mplane + mrow]);
         }
       }
-      double* o = out.p[t.x];
+      const gmut<double> o = gmem_w(out.p[t.x]);
       if (MODE == 0) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) o[v * cells + c] = r[v];
       } else if (MODE == 1) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) o[v * cells + c] = qi[v * cells + c] + dt * r[v];
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = vq[v] + dt * r[v];
       } else {
-        const double* qz = q0.p[t.x];
         const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
         const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
-          o[v * cells + c] = a0 * qz[v * cells + c] + a1 * (qi[v * cells + c] + dt * r[v]);
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = a0 * vz[v] + a1 * (vq[v] + dt * r[v]);
       }
     }
     lo ^= 1;
@@ -678,12 +712,14 @@ hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, in
   if (ntiles <= 0) return hipSuccess;
   dim3 grid(ntiles), block(NT);
   if (nd.prec == WMX_PREC_FP32_BULK) {
+#define WMX_F2(M) hipLaunchKernelGGL((k_flux2<float, M>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt)
     switch (mode) {
-      case 0: hipLaunchKernelGGL((k_flux2<float, 0>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
-      case 1: hipLaunchKernelGGL((k_flux2<float, 1>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
-      case 2: hipLaunchKernelGGL((k_flux2<float, 2>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
-      default: hipLaunchKernelGGL((k_flux2<float, 3>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt); break;
+      case 0: WMX_F2(0); break;
+      case 1: WMX_F2(1); break;
+      case 2: WMX_F2(2); break;
+      default: WMX_F2(3); break;
     }
+#undef WMX_F2
     return hipGetLastError();
   }
   switch (mode) {
