@@ -318,7 +318,7 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // register window costs about 1.5 planes, the FP64 kernel's about 0.5).  The shortest
 // estimate over ci in [4, 32] wins (ties: the longer chunk).  dims: n[3] per block.
 int auto_ci_len(int nb, const int* dims, int prec, int cu_count, long long* ntiles) {
-  const int tjr = prec == WMX_PREC_FP32_BULK ? FLUX2_TILE_J : 16;
+  const int tjr = FLUX2_TILE_J;
   const double alpha = prec == WMX_PREC_FP32_BULK ? 1.5 : 0.5;
   const long long slots = 2LL * std::max(1, cu_count);
   int best = 16;
@@ -339,8 +339,8 @@ int auto_ci_len(int nb, const int* dims, int prec, int cu_count, long long* ntil
 
 void build_tiles(wmx_plan* p) {
   std::vector<int4> t;
-  // tile rows: 16 for the FP64 flux kernel, 15 for the FP32-bulk one
-  const int tjr = p->nd.prec == WMX_PREC_FP32_BULK ? FLUX2_TILE_J : 16;
+  // tile rows of both flux kernels
+  const int tjr = FLUX2_TILE_J;
   if (p->ci_req > 0) {
     p->ci_len = p->ci_req;
   } else {

@@ -68,7 +68,7 @@ struct NumDev {
   WallNetDev net;
 };
 
-constexpr int FLUX2_TILE_J = 15;   // cell rows of an FP32-bulk flux tile (flux.hip)
+constexpr int FLUX2_TILE_J = 15;   // cell rows of a flux-kernel tile (flux.hip)
 
 struct GhostOp {
   int blk;       // destination block

@@ -270,6 +270,36 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I0 - 1) * sI + cb];
   const int own_s = tj * TK + tk;               // this column's i-face slot (own threads)
   int lo = 0;                                    // slot of the current lower i face
+  // halo ring job of this thread (plane-independent): 140 primitive slots (halo 2),
+  // 66 gradient/centroid slots (halo 1); prefetched one plane ahead into registers
+  constexpr int NRP = NP2 - C2J * TK, NRG = NG2 - C2J * TK;
+  int ring = 0, ring_s = 0;                      // 1: primitive slot, 2: gradient slot
+  long long ring_off = 0;
+  if (tid < NRP) {
+    int sj, sk;
+    ring_slot(tid, 2, C2J, &sj, &sk);
+    const int jj = ng + j0 - 2 + sj, kk = ng + k0 - 2 + sk;
+    if (p_ok(jj, kk)) { ring = 1; ring_s = sj * H2K + sk; ring_off = (long long)jj * N[2] + kk; }
+  } else if (tid < NRP + NRG) {
+    int sj, sk;
+    ring_slot(tid - NRP, 1, C2J, &sj, &sk);
+    const int jj = ng + j0 - 1 + sj, kk = ng + k0 - 1 + sk;
+    if (g_ok(jj, kk)) { ring = 2; ring_s = sj * G2K + sk; ring_off = (long long)jj * N[2] + kk; }
+  }
+  R rv[NGW];
+  double rx[3];
+  auto ring_load = [&](int Ip) {
+    const long long c = (long long)Ip * sI + ring_off;
+    if (ring == 1) {
+#pragma unroll
+      for (int i = 0; i < NPW; ++i) rv[i] = P[(i < 5 ? i : (i == 5 ? P_E : P_T)) * cells + c];
+    } else if (ring == 2) {
+#pragma unroll
+      for (int i = 0; i < NGW; ++i) rv[i] = G[i * cells + c];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) rx[x] = XC[x * cells + c];
+    }
+  };
   for (int I = I0 - 1; I < ng + i1; ++I) {
     const bool first = I == I0 - 1;
     // ---- shift the window (planes I-1..I+1 of primitives, I of gradients/centroids)
@@ -312,39 +342,15 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           for (int x = 0; x < 3; ++x) sX[x][sg] = vx[x];
         }
       }
-      // halo rings: 140 primitive slots (halo 2), 66 gradient/centroid slots (halo 1)
-      constexpr int NRP = NP2 - C2J * TK, NRG = NG2 - C2J * TK;
-      if (tid < NRP) {
-        int sj, sk;
-        ring_slot(tid, 2, C2J, &sj, &sk);
-        const int jj = ng + j0 - 2 + sj, kk = ng + k0 - 2 + sk;
-        if (p_ok(jj, kk)) {
-          const long long c = pl + (long long)jj * N[2] + kk;
-          const int s = sj * H2K + sk;
-          R vp[NPW];
+      // halo ring from the registers prefetched during the previous plane
+      if (ring == 1) {
 #pragma unroll
-          for (int i = 0; i < NPW; ++i) vp[i] = P[(i < 5 ? i : (i == 5 ? P_E : P_T)) * cells + c];
+        for (int i = 0; i < NPW; ++i) sP[i][ring_s] = rv[i];
+      } else if (ring == 2) {
 #pragma unroll
-          for (int i = 0; i < NPW; ++i) sP[i][s] = vp[i];
-        }
-      } else if (tid < NRP + NRG) {
-        int sj, sk;
-        ring_slot(tid - NRP, 1, C2J, &sj, &sk);
-        const int jj = ng + j0 - 1 + sj, kk = ng + k0 - 1 + sk;
-        if (g_ok(jj, kk)) {
-          const long long c = pl + (long long)jj * N[2] + kk;
-          const int s = sj * G2K + sk;
-          R vg[NGW];
-          double vx[3];
+        for (int i = 0; i < NGW; ++i) sG[i][ring_s] = rv[i];
 #pragma unroll
-          for (int i = 0; i < NGW; ++i) vg[i] = G[i * cells + c];
-#pragma unroll
-          for (int x = 0; x < 3; ++x) vx[x] = XC[x * cells + c];
-#pragma unroll
-          for (int i = 0; i < NGW; ++i) sG[i][s] = vg[i];
-#pragma unroll
-          for (int x = 0; x < 3; ++x) sX[x][s] = vx[x];
-        }
+        for (int x = 0; x < 3; ++x) sX[x][ring_s] = rx[x];
       }
     }
     // ---- load plane I+2 (primitives) and I+1 (gradients, centroids) into the window;
@@ -370,9 +376,19 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         fI = I + 1;
         L2 = Pw[0].w; L = Pw[1].w; Rc = Pw[2].w; R2 = Pw[3].w;
         TL = Pw[1].T; TR = Pw[2].T;
-        gL = Gw[0].g; gR = Gw[1].g; thL = Gw[0].th; thR = Gw[1].th;
+        gR = Gw[1].g; thR = Gw[1].th;
 #pragma unroll
-        for (int x = 0; x < 3; ++x) { xL[x] = Xw[0][x]; xR[x] = Xw[1][x]; }
+        for (int x = 0; x < 3; ++x) xR[x] = Xw[1][x];
+        if (first) {
+          gL = Gw[0].g; thL = Gw[0].th;
+#pragma unroll
+          for (int x = 0; x < 3; ++x) xL[x] = Xw[0][x];
+        } else {                                     // plane I of the own column, staged
+          const int sg = (tj + 1) * G2K + (tk + 1);
+          gL = sg_g<R>(sG, sg); thL = sG[14][sg];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) xL[x] = sX[x][sg];
+        }
       } else {
         if (task == 0) {
           d = 1;
@@ -434,7 +450,10 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         }
       }
     }
-    if (first) continue;                           // uniform: no barrier skipped
+    if (first) {                                   // uniform: no barrier skipped
+      ring_load(I + 1);
+      continue;
+    }
     // the epilogue's state and volume loads go out before the barrier
     const long long c = (long long)I * sI + (long long)J * N[2] + K;
     const gconst<double> qi = gmem(qin.p[t.x]);
@@ -443,11 +462,6 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
       ivol = gmem(B.vol)[c];
 #pragma unroll
       for (int v = 0; v < 5; ++v) vq[v] = qi[v * cells + c];
-      if (MODE >= 2) {
-        const gconst<double> qz = gmem(q0.p[t.x]);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) vz[v] = qz[v * cells + c];
-      }
     }
     __syncthreads();
     if (act) {
@@ -484,19 +498,24 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
 #pragma unroll
         for (int v = 0; v < 5; ++v) o[v * cells + c] = vq[v] + dt * r[v];
       } else {
+        const gconst<double> qz = gmem(q0.p[t.x]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) vz[v] = qz[v * cells + c];
         const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
         const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
 #pragma unroll
         for (int v = 0; v < 5; ++v) o[v * cells + c] = a0 * vz[v] + a1 * (vq[v] + dt * r[v]);
       }
     }
+    // halo ring of the next plane, in flight across the barrier
+    if (I + 1 < ng + i1) ring_load(I + 1);
     lo ^= 1;
     __syncthreads();
   }
 }
 
 // ------------------------------------------------------------------------------ flux
-__device__ __forceinline__ CellW load_w(const double* __restrict__ P, long long cells, long long c) {
+__device__ __forceinline__ CellW load_w(gconst<double> P, long long cells, long long c) {
   CellW w;
   w.r = P[c];
   w.u = P[cells + c];
@@ -507,7 +526,7 @@ __device__ __forceinline__ CellW load_w(const double* __restrict__ P, long long 
   return w;
 }
 
-__device__ __forceinline__ CellG load_g(const double* __restrict__ G, long long cells, long long c) {
+__device__ __forceinline__ CellG load_g(gconst<double> G, long long cells, long long c) {
   CellG g;
 #pragma unroll
   for (int i = 0; i < 12; ++i) g.g[i] = G[i * cells + c];
@@ -531,7 +550,7 @@ __device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, i
     tangents(d, &t1, &t2);
     const long long nw = (long long)B.n[t1] * B.n[t2];
     const long long w = (long long)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
-    const double* W = B.wall[2 * d + side];
+    const gconst<double> W = gmem(B.wall[2 * d + side]);
 #pragma unroll
     for (int v = 0; v < 5; ++v) F[v] = W[v * nw + w];
     return;
@@ -540,10 +559,10 @@ __device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, i
   const long long cells = B.cells;
   const long long sd = d == 0 ? (long long)N[1] * N[2] : (d == 1 ? (long long)N[2] : 1LL);
   const long long cR = cidx(N, I, J, K), cL = cR - sd;
-  const double* P = B.P;
-  const double* G = B.G;
+  const gconst<double> P = gmem(B.P);
+  const gconst<double> G = gmem(B.G);
   const long long fi = fidx(N, d, I, J, K), fc = fcount(N, d);
-  const double* S = B.S[d];
+  const gconst<double> S = gmem(B.S[d]);
   const double Sx = S[fi], Sy = S[fc + fi], Sz = S[2 * fc + fi];
   const CellW L = load_w(P, cells, cL), R = load_w(P, cells, cR);
   const int sch = nd.scheme;
@@ -575,21 +594,26 @@ __device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, i
   const CellG gL = load_g(G, cells, cL), gR = load_g(G, cells, cR);
   const double phL[4] = {L.u, L.v, L.w, P[P_T * cells + cL]};
   const double phR[4] = {R.u, R.v, R.w, P[P_T * cells + cR]};
-  const double xL[3] = {B.xc[cL], B.xc[cells + cL], B.xc[2 * cells + cL]};
-  const double xR[3] = {B.xc[cR], B.xc[cells + cR], B.xc[2 * cells + cR]};
+  const gconst<double> XC = gmem(B.xc);
+  const double xL[3] = {XC[cL], XC[cells + cL], XC[2 * cells + cL]};
+  const double xR[3] = {XC[cR], XC[cells + cR], XC[2 * cells + cR]};
   viscous_face(gL, gR, phL, phR, xL, xR, Sx, Sy, Sz, F);
 }
 
 // FP64 plain form: every face reads its stencil through the caches (no register
-// window), which keeps the kernel at 198 VGPRs / 2 waves per SIMD.
+// window), which keeps the kernel near 200 VGPRs / 2 waves per SIMD.  Tiles of 15 x 16
+// cells on a 16 x 16 thread grid, as in k_flux2: per plane thread (tj, tk) evaluates the
+// j face below row tj, the k face left of its cell (tj < 15) or the upper edge k face of
+// row tk (tj = 15), and the i face above its cell (tj < 15) -- three face tasks for every
+// wavefront.
 // MODE 0: r = RHS; 1: q + dt r; 2: 3/4 q0 + 1/4 (q + dt r); 3: 1/3 q0 + 2/3 (q + dt r)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks,
                                               const int4* __restrict__ tiles, int ci_len,
                                               NumDev nd, Ptrs qin, Ptrs q0, Ptrs out,
                                               const double* __restrict__ d_dt, double dt_host) {
-  __shared__ double sFj[5][(TJ + 1) * TK];
-  __shared__ double sFk[5][TJ * (TK + 1)];
+  __shared__ double sFj[5][(C2J + 1) * TK];
+  __shared__ double sFk[5][C2J * (TK + 1)];
   const int4 t = tiles[blockIdx.x];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
@@ -597,58 +621,59 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
   const int tid = threadIdx.x;
   const int tj = tid / TK, tk = tid % TK;
   const int j = j0 + tj, k = k0 + tk;
-  const bool act = j < B.n[1] && k < B.n[2];
-  const int jl = min(j0 + TJ, B.n[1]) - j0;
+  const bool own = tj < C2J;
+  const bool act = own && j < B.n[1] && k < B.n[2];
+  const int jl = min(j0 + C2J, B.n[1]) - j0;
   const int kl = min(k0 + TK, B.n[2]) - k0;
   const int ng = B.ng;
   const int J = ng + j, K = ng + k;
   const double dt = d_dt ? d_dt[B.case_id] : dt_host;
   const long long cells = B.cells;
   double Flo[5], Fhi[5];
+  if (act) face_flux(nd, B, 0, ng + i0, J, K, Flo);
   for (int i = i0; i < i1; ++i) {
     const int I = ng + i;
 #pragma unroll 1
-    for (int task = (i == i0 ? 0 : 1); task < 5; ++task) {
-      if (task == 4 && tid >= 64) break;                 // tile-edge faces: wave 0 only
-      int d = 0, fI = I, fJ = J, fK = K;
-      bool doit = act;
-      if (task == 0) { d = 0; }
-      else if (task == 1) { d = 0; fI = I + 1; }
-      else if (task == 2) { d = 1; }
-      else if (task == 3) { d = 2; }
-      else {
-        if (tid < TK) { d = 1; fJ = ng + j0 + jl; fK = ng + k0 + tid; doit = tid < kl; }
-        else if (tid < TK + TJ) { d = 2; fJ = ng + j0 + (tid - TK); fK = ng + k0 + kl; doit = (tid - TK) < jl; }
-        else doit = false;
-      }
+    for (int task = 0; task < 3; ++task) {
+      int d, fI = I, rj = tj, rk = tk;
+      bool doit;
+      if (task == 0) { d = 1; doit = rj <= jl && rk < kl; }
+      else if (task == 1) {
+        d = 2;
+        if (own) doit = rj < jl && rk < kl;
+        else { rj = tk; rk = kl; doit = rj < jl; }
+      } else { d = 0; fI = I + 1; doit = act; }
       double F[5];
-      if (doit) face_flux(nd, B, d, fI, fJ, fK, F);
       if (doit) {
+        face_flux(nd, B, d, fI, ng + j0 + rj, ng + k0 + rk, F);
         if (task == 0) {
 #pragma unroll
-          for (int v = 0; v < 5; ++v) Flo[v] = F[v];
+          for (int v = 0; v < 5; ++v) sFj[v][rj * TK + rk] = F[v];
         } else if (task == 1) {
 #pragma unroll
-          for (int v = 0; v < 5; ++v) Fhi[v] = F[v];
-        } else if (task == 2) {
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sFj[v][tj * TK + tk] = F[v];
-        } else if (task == 3) {
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sFk[v][tj * (TK + 1) + tk] = F[v];
-        } else if (tid < TK) {
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sFj[v][jl * TK + tid] = F[v];
+          for (int v = 0; v < 5; ++v) sFk[v][rj * (TK + 1) + rk] = F[v];
         } else {
 #pragma unroll
-          for (int v = 0; v < 5; ++v) sFk[v][(tid - TK) * (TK + 1) + kl] = F[v];
+          for (int v = 0; v < 5; ++v) Fhi[v] = F[v];
         }
+      }
+    }
+    // the epilogue's state and volume loads go out before the barrier
+    const long long c = cidx(B.N, I, J, K);
+    const gconst<double> qi = gmem(qin.p[t.x]);
+    double vq[5], vz[5], vol = 1.0;
+    if (act) {
+      vol = gmem(B.vol)[c];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) vq[v] = qi[v * cells + c];
+      if (MODE >= 2) {
+        const gconst<double> qz = gmem(q0.p[t.x]);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) vz[v] = qz[v * cells + c];
       }
     }
     __syncthreads();
     if (act) {
-      const long long c = cidx(B.N, I, J, K);
-      const double vol = B.vol[c];
       double r[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -657,10 +682,9 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
         a = a - (sFk[v][tj * (TK + 1) + tk + 1] - sFk[v][tj * (TK + 1) + tk]);
         r[v] = a / vol;
       }
-      const double* qi = qin.p[t.x];
       if (nd.n_sponge > 0) {
         // SmoothRampSponge, smooth_ramp.py:212-221
-        const double xcv = B.xc[c];
+        const double xcv = gmem(B.xc)[c];
         const long long mrow = (long long)i * B.n[1] + j;
         const long long mplane = (long long)B.n[0] * B.n[1];
         for (int sp = 0; sp < nd.n_sponge; ++sp) {
@@ -671,23 +695,21 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
           double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
 #pragma unroll
           for (int v = 0; v < 5; ++v)
-            r[v] = r[v] + (-sigma) * (qi[v * cells + c] - B.sponge_mean[v * mplane + mrow]);
+            r[v] = r[v] + (-sigma) * (vq[v] - gmem(B.sponge_mean)[v * mplane + mrow]);
         }
       }
-      double* o = out.p[t.x];
+      const gmut<double> o = gmem_w(out.p[t.x]);
       if (MODE == 0) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) o[v * cells + c] = r[v];
       } else if (MODE == 1) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) o[v * cells + c] = qi[v * cells + c] + dt * r[v];
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = vq[v] + dt * r[v];
       } else {
-        const double* qz = q0.p[t.x];
         const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
         const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
-          o[v * cells + c] = a0 * qz[v * cells + c] + a1 * (qi[v * cells + c] + dt * r[v]);
+        for (int v = 0; v < 5; ++v) o[v * cells + c] = a0 * vz[v] + a1 * (vq[v] + dt * r[v]);
       }
 #pragma unroll
       for (int v = 0; v < 5; ++v) Flo[v] = Fhi[v];

@@ -50,9 +50,10 @@ __device__ __forceinline__ long long interior_cell(const BlockDev& B, long long 
 __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__ blks,
                                                     const int* __restrict__ wg_prefix, int nblk,
                                                     Ptrs q, NumDev nd, double* __restrict__ part) {
-  const int b = find_block_r(wg_prefix, nblk, blockIdx.x);
+  const int wg = blockIdx.x;
+  const int b = find_block_r(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
-  const long long l = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const long long l = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
   const long long nint = (long long)B.n[0] * B.n[1] * B.n[2];
   double dtc = __builtin_huge_val();
   if (l < nint) {
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
     dtc = nd.cfl / ((lam + 4.0 * lamv) / vol);
   }
   const double r = block_reduce<true>(dtc);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
+  if (threadIdx.x == 0) part[wg] = r;
 }
 
 // sum over interiors of rho u_dir vol (and vol); solver.py:23-27

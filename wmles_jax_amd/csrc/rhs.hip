@@ -72,11 +72,12 @@ template <class R>
 __global__ __launch_bounds__(256) void k_grad(const BlockDev* __restrict__ blks,
                                               const int* __restrict__ wg_prefix, int nblk,
                                               NumDev nd) {
-  const int b = find_block(wg_prefix, nblk, blockIdx.x);
+  const int wg = blockIdx.x;
+  const int b = find_block(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
   const int ng = B.ng;
   const int b0 = B.n[0] + 2, b1 = B.n[1] + 2, b2 = B.n[2] + 2;
-  long long l = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  long long l = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
   if (l >= (long long)b0 * b1 * b2) return;
   const int K = ng - 1 + (int)(l % b2);
   const int J = ng - 1 + (int)((l / b2) % b1);
