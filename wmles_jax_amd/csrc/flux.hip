@@ -507,10 +507,13 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         for (int v = 0; v < 5; ++v) o[v * cells + c] = a0 * vz[v] + a1 * (vq[v] + dt * r[v]);
       }
     }
-    // halo ring of the next plane, in flight across the barrier
+    // halo ring of the next plane, in flight while the next plane's staging waits.  No
+    // barrier here: the next staging only overwrites what this plane's face tasks read
+    // (all done before the barrier above), and the next face tasks write the exchange
+    // buffers only after the next staging barrier, which every thread reaches after
+    // its epilogue.
     if (I + 1 < ng + i1) ring_load(I + 1);
     lo ^= 1;
-    __syncthreads();
   }
 }
 
