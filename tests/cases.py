@@ -104,9 +104,10 @@ def channel_props(Re_b=2.0e4, Ma_b=0.3, T0=300.0, rho0=1.2):
 
 
 def channel_spec(nblocks_x, nx, ny, nz, scheme, sgs, T_wall=300.0, Re_b=2.0e4, beta=2.0,
-                 name="channel", mass_flux=True, splits=None):
+                 name="channel", mass_flux=True, splits=None, n_match=1):
     """Wall-modelled channel 2 pi d x 2 d x pi d, `nblocks_x` blocks in x (periodic
-    x/z, WallModel on jmin/jmax, constant mass flux).  `nx` is the total x count."""
+    x/z, WallModel on jmin/jmax sampling the `n_match`-th cell off the wall, constant mass
+    flux).  `nx` is the total x count."""
     pr = channel_props(Re_b=Re_b)
     d = pr["d"]
     Lx, Lz = 2 * np.pi * d, np.pi * d
@@ -131,7 +132,7 @@ def channel_spec(nblocks_x, nx, ny, nz, scheme, sgs, T_wall=300.0, Re_b=2.0e4, b
     for nm in blocks:
         connect.append([nm, "kmin", nm, "kmax", [0.0, 0.0, -Lz]])
         for f in ("jmin", "jmax"):
-            bcs.append([nm, f, {"kind": "WallModel", "T_wall": T_wall, "n_match": 1}])
+            bcs.append([nm, f, {"kind": "WallModel", "T_wall": T_wall, "n_match": n_match}])
     cons = [{"kind": "ConstantMassFlux", "direction": 0, "target": pr["rho0"] * pr["Ub"]}] \
         if mass_flux else []
     spec = {"name": name, "blocks": blocks, "ng": 3, "connect": connect, "bcs": bcs,

@@ -579,7 +579,8 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   std::vector<long long> c_all, c_box, c_int, c_shell;
   for (auto& B : p->hb) {
     c_all.push_back(B.cells);
-    c_box.push_back((long long)(B.n[0] + 2) * (B.n[1] + 2) * (B.n[2] + 2));
+    const int run = grad_run(nd.prec == WMX_PREC_FP32_BULK ? sizeof(float) : sizeof(double));
+    c_box.push_back((long long)((B.n[0] + 2 + run - 1) / run) * (B.n[1] + 2) * (B.n[2] + 2));
     c_int.push_back((long long)B.n[0] * B.n[1] * B.n[2]);
     c_shell.push_back(B.shell_cells);
   }

@@ -69,6 +69,8 @@ struct NumDev {
 };
 
 constexpr int FLUX2_TILE_J = 15;   // cell rows of a flux-kernel tile (flux.hip)
+// i cells per thread of the gradient kernel (rhs.hip): 4 in FP32, 1 in FP64
+constexpr int grad_run(size_t rec_bytes) { return rec_bytes == 4 ? 4 : 1; }
 
 struct GhostOp {
   int blk;       // destination block

@@ -68,8 +68,9 @@ __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blk
 }
 
 // ------------------------------------------------------------------------------ grad
+// FP64: one thread per cell.
 template <class R>
-__global__ __launch_bounds__(256) void k_grad(const BlockDev* __restrict__ blks,
+__global__ __launch_bounds__(256) void k_grad_cell(const BlockDev* __restrict__ blks,
                                               const int* __restrict__ wg_prefix, int nblk,
                                               NumDev nd) {
   const int wg = blockIdx.x;
@@ -147,8 +148,127 @@ __global__ __launch_bounds__(256) void k_grad(const BlockDev* __restrict__ blks,
   G[G_TH * cells + c] = ducros<R>(g);
 }
 
-template __global__ void k_grad<double>(const BlockDev*, const int*, int, NumDev);
-template __global__ void k_grad<float>(const BlockDev*, const int*, int, NumDev);
+// FP32-bulk:// One thread per run of grad_run() consecutive i cells of one (j, k) column: the i
+// neighbours' phi and the shared i-face vectors stay in registers from one cell of the
+// run to the next, so the planes i-1 / i+1 (far apart in memory) are read about once
+// instead of three times.  Per cell the arithmetic is that of a single-cell evaluation.
+template <class R>
+__global__ __launch_bounds__(256) void k_grad_run(const BlockDev* __restrict__ blks,
+                                              const int* __restrict__ wg_prefix, int nblk,
+                                              NumDev nd) {
+  const int wg = blockIdx.x;
+  const int b = find_block(wg_prefix, nblk, wg);
+  const BlockDev& B = blks[b];
+  const int ng = B.ng;
+  const int b0 = B.n[0] + 2, b1 = B.n[1] + 2, b2 = B.n[2] + 2;
+  constexpr int RUN = grad_run(sizeof(R));
+  const int nrun = (b0 + RUN - 1) / RUN;
+  const long long l = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  if (l >= (long long)nrun * b1 * b2) return;
+  const int K = ng - 1 + (int)(l % b2);
+  const int J = ng - 1 + (int)((l / b2) % b1);
+  const int Ia = ng - 1 + (int)(l / ((long long)b1 * b2)) * RUN;
+  const int Ib = min(Ia + RUN, ng + B.n[0] + 1);
+  const int* N = B.N;
+  const long long cells = B.cells;
+  const long long st[3] = {(long long)N[1] * N[2], (long long)N[2], 1};
+  const gconst<R> P = gmem(sizeof(R) == 8 ? (const R*)B.P : (const R*)B.Pf);
+  const gmut<R> G = gmem_w(sizeof(R) == 8 ? (R*)B.G : (R*)B.Gf);
+  const int pl[4] = {P_U, P_V, P_W, P_T};
+  const R h = R(0.5);
+  const GasC& gs = nd.gas;
+  // window: phi of planes I-1 and I, the i-face vector of face I
+  R phm[4], ph0[4], S0lo[3];
+  {
+    const long long c = cidx(N, Ia, J, K);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      phm[v] = P[pl[v] * cells + c - st[0]];
+      ph0[v] = P[pl[v] * cells + c];
+    }
+    const gconst<double> S = gmem(B.S[0]);
+    const long long fc = fcount(N, 0), f = fidx(N, 0, Ia, J, K);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) S0lo[x] = R(S[x * fc + f]);
+  }
+#pragma unroll 1
+  for (int I = Ia; I < Ib; ++I) {
+    const long long c = cidx(N, I, J, K);
+    R php[4], S0hi[3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) php[v] = P[pl[v] * cells + c + st[0]];
+    {
+      const gconst<double> S = gmem(B.S[0]);
+      const long long fc = fcount(N, 0), f = fidx(N, 0, I + 1, J, K);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) S0hi[x] = R(S[x * fc + f]);
+    }
+    R acc[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) acc[i] = R(0.0);
+    int cc[3] = {I, J, K};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R Slo[3], Shi[3];
+      if (d == 0) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { Slo[x] = S0lo[x]; Shi[x] = S0hi[x]; }
+      } else {
+        const gconst<double> S = gmem(B.S[d]);
+        const long long fc = fcount(N, d);
+        int hi[3] = {cc[0], cc[1], cc[2]};
+        hi[d] += 1;
+        const long long flo = fidx(N, d, cc[0], cc[1], cc[2]);
+        const long long fhi = fidx(N, d, hi[0], hi[1], hi[2]);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) { Slo[x] = R(S[x * fc + flo]); Shi[x] = R(S[x * fc + fhi]); }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const R pm = d == 0 ? phm[v] : P[pl[v] * cells + c - st[d]];
+        const R pp = d == 0 ? php[v] : P[pl[v] * cells + c + st[d]];
+        const R flo_v = h * (pm + ph0[v]);
+        const R fhi_v = h * (ph0[v] + pp);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
+      }
+    }
+    const double vol = gmem(B.vol)[c];
+    const R volr = R(vol);
+    R g[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) g[i] = fdiv<R>(acc[i], volr);
+    const R rho = P[c];
+    const R mu = P[P_MU * cells + c];
+    R mut = R(0.0);
+    if (nd.sgs != WMX_SGS_NONE) {
+      // filter width vol^(1/3) (solver.py:82); FP32-bulk: single-precision cube root
+      R delta;
+      if constexpr (sizeof(R) == 4) delta = cbrtf((float)vol);
+      else delta = R(pow(vol, 1.0 / 3.0));
+      const R cs = R(nd.sgs_c);
+      R nut;
+      if (nd.sgs == WMX_SGS_SMAGORINSKY) nut = nut_smagorinsky<R>(g, delta, cs);
+      else if (nd.sgs == WMX_SGS_WALE) nut = nut_wale<R>(g, delta, cs);
+      else nut = nut_vreman<R>(g, delta, cs);
+      mut = rho * nut;
+    } else {
+      mut = R(0.0) * rho;
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) G[i * cells + c] = g[i];
+    G[G_MU * cells + c] = mu + mut;
+    G[G_K * cells + c] = R(gs.cp) * (fdiv<R>(mu, R(gs.Pr)) + fdiv<R>(mut, R(gs.Pr_t)));
+    G[G_TH * cells + c] = ducros<R>(g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) { phm[v] = ph0[v]; ph0[v] = php[v]; }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) S0lo[x] = S0hi[x];
+  }
+}
+
+template __global__ void k_grad_cell<double>(const BlockDev*, const int*, int, NumDev);
+template __global__ void k_grad_run<float>(const BlockDev*, const int*, int, NumDev);
 
 // ------------------------------------------------------------------------------ wall
 // NET: the learned model (wave-cooperative network + Newton polish) replaces the
@@ -317,9 +437,9 @@ hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int n
                        hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   if (nd.prec == WMX_PREC_FP32_BULK)
-    hipLaunchKernelGGL(k_grad<float>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
+    hipLaunchKernelGGL(k_grad_run<float>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
   else
-    hipLaunchKernelGGL(k_grad<double>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
+    hipLaunchKernelGGL(k_grad_cell<double>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
   return hipGetLastError();
 }
 
