@@ -317,7 +317,7 @@ def bench_wm(args, wmx):
     flux_ms, flux_n = prof_l["flux"]
     rec = _record(args, cells * args.steps / el_l / 1e6, el_l * 1e3 / args.steps, "f32-bulk+f64", workload,
                   {"cells": cells, "blocks": 1, "form": "fp32", "wall_faces": n_faces,
-                   "network": "3-64-64-2 tanh, BF16 MFMA inference, 3 Newton polish steps",
+                   "network": "3-64-64-2 ISRU, BF16 MFMA inference, 3 Newton polish steps",
                    "n_match": n_match},
                   _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
     rec["wall_model"] = {

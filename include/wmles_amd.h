@@ -119,7 +119,7 @@ typedef struct wmx_inflow_desc {
 
 /* Learned wall model (no reference code: it plugs in at the wall-model callable
    boundary model(gas, h, U, T_h, p_h, T_wall), reference wmles/wall_model.py:3-9,
-   called at wmles/bc.py:161).  MLP 3 -> 64 -> 64 -> 2 with tanh hidden layers;
+   called at wmles/bc.py:161).  MLP 3 -> 64 -> 64 -> 2, hidden activation a / sqrt(1 + a^2);
    inputs log Re_h, h/Delta, T_h/T_w; outputs log u+ and log T+ (normalised).
    FP32 master parameters (wmx_mlp_sizes: nparam) are ordered
    W1[64][3] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]; wmx_mlp_pack derives the

@@ -11,7 +11,8 @@ safeguarded fixed-count Newton solve for Spalding's law and Kader's temperature 
   features uniform over the trained box, targets log u+ and log T+ from the Newton
   oracle).  Pinned by: u+ satisfies Spalding's law (residual), T+ equals Kader's
   closed form, the hash's first outputs against the published splitmix64 sequence.
-* `forward`, `loss_grad`: the 3 -> 64 -> 64 -> 2 tanh network and the gradient of the
+* `forward`, `loss_grad`: the 3 -> 64 -> 64 -> 2 network with the ISRU activation
+  a / sqrt(1 + a^2) and the gradient of the
   mean squared error in float64.  Pinned by central finite differences of the loss
   and by torch.autograd (float64, CPU) on the same parameters.
 * `adam`: Adam (Kingma & Ba 2015, Algorithm 1) with beta1 0.9, beta2 0.999, eps 1e-8.
@@ -82,12 +83,22 @@ def dataset(n, seed, lo, hi, x_center, x_scale, y_center, y_scale, Pr=0.72, wm=N
     return X, Y, raw
 
 
+def act(a):
+    """Hidden activation a / sqrt(1 + a^2)."""
+    return a / np.sqrt(1.0 + a * a)
+
+
+def act_prime(h):
+    """Its derivative written in the activation h: (1 - h^2)^(3/2) = (1 + a^2)^(-3/2)."""
+    return (1.0 - h * h) ** 1.5
+
+
 def forward(P, X):
     """Network outputs (n, 2) in float64 and the hidden activations."""
     W1, b1, W2, b2, W3, b3 = unpack(P)
     X = np.asarray(X, dtype=np.float64)
-    H1 = np.tanh(X @ W1.T + b1)
-    H2 = np.tanh(H1 @ W2.T + b2)
+    H1 = act(X @ W1.T + b1)
+    H2 = act(H1 @ W2.T + b2)
     return H2 @ W3.T + b3, (X, H1, H2)
 
 
@@ -100,8 +111,8 @@ def loss_grad(P, X, Y):
     E = out - Y
     loss = float(np.sum(E * E) / (n * NOUT))
     dY = 2.0 * E / (n * NOUT)
-    D2 = (dY @ W3) * (1.0 - H2 * H2)
-    D1 = (D2 @ W2) * (1.0 - H1 * H1)
+    D2 = (dY @ W3) * act_prime(H2)
+    D1 = (D2 @ W2) * act_prime(H1)
     g = np.zeros(NPARAM)
     g[O_W1:O_B1] = (D1.T @ X).ravel()
     g[O_B1:O_W2] = D1.sum(axis=0)

@@ -122,11 +122,23 @@ def test_gradient_matches_torch_autograd():
     W3 = Pt[L.O_W3:L.O_B3].view(2, 64)
     b3 = Pt[L.O_B3:]
     Xt, Yt = torch.tensor(X), torch.tensor(Y)
-    out = torch.tanh(torch.tanh(Xt @ W1.T + b1) @ W2.T + b2) @ W3.T + b3
+    act = lambda a: a * torch.rsqrt(1.0 + a * a)
+    out = act(act(Xt @ W1.T + b1) @ W2.T + b2) @ W3.T + b3
     lt = torch.mean((out - Yt) ** 2)
     lt.backward()
     assert abs(float(lt.detach()) - loss) < 1e-14
     np.testing.assert_allclose(g, Pt.grad.numpy(), rtol=1e-10, atol=1e-14)
+
+
+def test_activation_and_derivative():
+    a = np.linspace(-6, 6, 241)
+    h = L.act(a)
+    np.testing.assert_allclose(h, a / np.sqrt(1 + a * a), rtol=1e-15)
+    assert np.all(np.abs(h) < 1) and np.all(np.diff(h) > 0)           # bounded, increasing
+    eps = 1e-5                                    # central difference: O(eps^2) + roundoff/eps
+    fd = (L.act(a + eps) - L.act(a - eps)) / (2 * eps)
+    np.testing.assert_allclose(L.act_prime(h), fd, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(L.act_prime(h), (1 + a * a) ** -1.5, rtol=1e-12)
 
 
 def test_adam_first_step_is_signed_rate():

@@ -49,6 +49,14 @@ __global__ void k_wm_dataset(NumDev nd, long long n, unsigned long long seed,
   }
 }
 
+// hidden activation (training and the FP32 forward): a / sqrt(1 + a^2); its derivative in
+// terms of the activation h is (1 - h^2)^(3/2)
+__device__ __forceinline__ float act32(float a) { return a / sqrtf(1.f + a * a); }
+__device__ __forceinline__ float act32_prime(float h) {
+  const float u = 1.f - h * h;
+  return u * sqrtf(u);
+}
+
 // ---------------------------------------------------------------------------- train
 // 64 samples per workgroup of 4 waves.  Lane = sample, wave w owns hidden units
 // w, w+4, ..., so weight reads are LDS broadcasts and activation reads walk rows of
@@ -79,7 +87,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ P,
   for (int j = w; j < MLP_H; j += 4) {
     float a = sP[MO_B1 + j];
     for (int i = 0; i < MLP_IN; ++i) a += sP[MO_W1 + j * MLP_IN + i] * sX[q * 3 + i];
-    sH1[q * TLD + j] = tanhf(a);
+    sH1[q * TLD + j] = act32(a);
   }
   __syncthreads();
   // layer 2
@@ -87,7 +95,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ P,
     float a = sP[MO_B2 + j];
 #pragma unroll 16
     for (int k = 0; k < MLP_H; ++k) a += sP[MO_W2 + j * MLP_H + k] * sH1[q * TLD + k];
-    sH2[q * TLD + j] = tanhf(a);
+    sH2[q * TLD + j] = act32(a);
   }
   __syncthreads();
   // output layer, d loss / d y
@@ -106,8 +114,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ P,
   for (int k = w; k < MLP_H; k += 4) {
     float g = 0.f;
     for (int o = 0; o < MLP_OUT; ++o) g += sP[MO_W3 + o * MLP_H + k] * sDY[q * 2 + o];
-    const float h = sH2[q * TLD + k];
-    sD2[q * TLD + k] = g * (1.f - h * h);
+    sD2[q * TLD + k] = g * act32_prime(sH2[q * TLD + k]);
   }
   __syncthreads();
   // back through layer 1's activation
@@ -115,8 +122,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(const float* __restrict__ P,
     float g = 0.f;
 #pragma unroll 16
     for (int j = 0; j < MLP_H; ++j) g += sP[MO_W2 + j * MLP_H + k] * sD2[q * TLD + j];
-    const float h = sH1[q * TLD + k];
-    sD1[q * TLD + k] = g * (1.f - h * h);
+    sD1[q * TLD + k] = g * act32_prime(sH1[q * TLD + k]);
   }
   __syncthreads();
   // weight gradients summed over the group's samples (invalid samples carry zero deltas)
@@ -207,14 +213,14 @@ __global__ __launch_bounds__(256) void k_mlp_forward32(const float* __restrict__
   float h1[MLP_H];
 #pragma unroll
   for (int j = 0; j < MLP_H; ++j)
-    h1[j] = tanhf(sP[MO_B1 + j] + sP[MO_W1 + 3 * j] * x0 + sP[MO_W1 + 3 * j + 1] * x1 +
+    h1[j] = act32(sP[MO_B1 + j] + sP[MO_W1 + 3 * j] * x0 + sP[MO_W1 + 3 * j + 1] * x1 +
                   sP[MO_W1 + 3 * j + 2] * x2);
   float y0 = sP[MO_B3], y1 = sP[MO_B3 + 1];
   for (int j = 0; j < MLP_H; ++j) {
     float a = sP[MO_B2 + j];
 #pragma unroll
     for (int k = 0; k < MLP_H; ++k) a += sP[MO_W2 + j * MLP_H + k] * h1[k];
-    const float h = tanhf(a);
+    const float h = act32(a);
     y0 += sP[MO_W3 + j] * h;
     y1 += sP[MO_W3 + MLP_H + j] * h;
   }

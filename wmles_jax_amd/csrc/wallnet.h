@@ -1,6 +1,7 @@
 // Learned equilibrium wall model: inference on one wavefront with BF16 MFMA.
 //
-// Network 3 -> 64 -> 64 -> 2, tanh hidden layers.  Inputs (normalised): log Re_h, h/Delta,
+// Network 3 -> 64 -> 64 -> 2, hidden activation ISRU f(a) = a / sqrt(1 + a^2) (one hardware
+// reciprocal square root per unit).  Inputs (normalised): log Re_h, h/Delta,
 // T_h/T_w.  Outputs (normalised): log u+ (u+ = U/u_tau) and log T+ (Kader's T+ at
 // y+ = Re_h/u+), from which u_tau and q_w follow exactly as in the equilibrium model
 // (reference wall_model.py:42-61).
@@ -38,9 +39,9 @@ __device__ __forceinline__ unsigned short f2bf(float f) {   // round to nearest 
   return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ float tanh_fast(float x) {
-  const float e = __expf(-2.f * fabsf(x));
-  return copysignf(__fdividef(1.f - e, 1.f + e), x);
+// hidden activation of the network (inference): a / sqrt(1 + a^2) with v_rsq_f32
+__device__ __forceinline__ float act_fast(float a) {
+  return a * __builtin_amdgcn_rsqf(fmaf(a, a, 1.f));
 }
 
 __device__ __forceinline__ f32x4 mfma_bf16_16x16x16(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -102,7 +103,7 @@ __device__ __forceinline__ void net_eval(const unsigned short* __restrict__ sW,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const float* w = w1[4 * c + v];
-        h1[c][v] = (short)f2bf(tanh_fast(w[3] + w[0] * x0 + w[1] * x1 + w[2] * x2));
+        h1[c][v] = (short)f2bf(act_fast(w[3] + w[0] * x0 + w[1] * x1 + w[2] * x2));
       }
     bf16x4 h2[4];
 #pragma unroll
@@ -111,7 +112,7 @@ __device__ __forceinline__ void net_eval(const unsigned short* __restrict__ sW,
 #pragma unroll
       for (int c = 0; c < 4; ++c) d = mfma_bf16_16x16x16(a2[m][c], h1[c], d);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) h2[m][v] = (short)f2bf(tanh_fast(d[v] + b2[4 * m + v]));
+      for (int v = 0; v < 4; ++v) h2[m][v] = (short)f2bf(act_fast(d[v] + b2[4 * m + v]));
     }
     f32x4 d = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

@@ -5,7 +5,8 @@ It plugs in at the reference's wall-model callable boundary
 at `wmles/bc.py:161`): pass it as `WallModel(LearnedWallModel(...), ...)` and the
 solver's wall-flux kernel evaluates it in place of the Newton solve.
 
-Network 3 -> 64 -> 64 -> 2 with tanh hidden layers.  Inputs: log Re_h, h/Delta
+Network 3 -> 64 -> 64 -> 2, hidden activation a / sqrt(1 + a^2) (ISRU: one hardware
+reciprocal square root per unit).  Inputs: log Re_h, h/Delta
 (Delta = vol^(1/3) of the sampled cell), T_h/T_w; outputs log u+ (u+ = U/u_tau) and
 log T+ (Kader's T+ at y+ = Re_h/u+), both normalised.  Everything runs in HIP kernels
 of libwmles_amd.so:
