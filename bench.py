@@ -259,7 +259,7 @@ def bench_ensemble(args, wmx):
 
     el_s, Qseq = _timed_steps(seq_step, Qs0, args.steps, args.warmup)
     same = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qseq) for a, b in zip(Qa, Qb))
-    workload = f"ensemble of 16 uneven channel cases ({nblk} blocks) Hybrid(KEP4+MUSCL-HLLC) Vreman WM fp32"
+    workload = f"ensemble of 16 uneven channel cases ({nblk} blocks) KEP2 WALE WM fp32"
     flux_ms, flux_n = prof["flux"]
     rec = _record(args, cells * args.steps / el_e / 1e6, el_e * 1e3 / args.steps, "f32-bulk+f64", workload,
                   {"cells": cells, "cases": 16, "blocks": nblk, "form": "fp32",
@@ -268,6 +268,7 @@ def bench_ensemble(args, wmx):
     rec["sequential_value"] = round(cells * args.steps / el_s / 1e6, 2)
     rec["ensemble_speedup_vs_sequential"] = round(el_s / el_e, 3)
     rec["ensemble_equals_sequential"] = same
+    rec["state_finite"] = all(bool(torch.isfinite(q).all()) for Q in Qe for q in Q)
     rec["autotune_flux_ms"] = {k: round(v, 4) for k, v in timings.items()}
     return rec
 

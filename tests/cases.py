@@ -158,11 +158,12 @@ def channel_prim(xc, props, seed=0, noise=0.05, interior=None):
     return np.stack([rho, vel[0], vel[1], vel[2], p])
 
 
-def ramp_spec(ns, nx=512, ny=128, nz=64, first_dy=1e-3, n_modes=64, seed=3, cfl=0.5):
+def ramp_spec(ns, nx=512, ny=128, nz=64, first_dy=1e-3, n_modes=64, seed=0, cfl=0.5):
     """Synthetic smooth-ramp block: x/L in [-0.76, 4], wall from y/L = 0.22 to 0 by a
     quintic over x/L in [0, 1], top at y/L = 0.74, one-sided tanh wall-normal stretching
     with first spacing `first_dy` L at the inflow, periodic span 0.288 L; Ma 0.2,
-    Re_L 667000; 1/7-power inflow with Fourier-mode fluctuations of 8% U rms;
+    Re_L 667000; 1/7-power inflow with Fourier-mode fluctuations of 8% U rms
+    (uu = vv = ww = 0.08^2, uv = 0);
     wall-modelled (adiabatic) bottom, slip top, characteristic outflow, sponge over the
     last 1 L.  Returns (spec, arrays, prim_fn)."""
     props = _resolve(ns, "flow_properties")(0.2, 667000.0)
@@ -191,7 +192,7 @@ def ramp_spec(ns, nx=512, ny=128, nz=64, first_dy=1e-3, n_modes=64, seed=3, cfl=
     yt = 0.22 + 0.52 * np.linspace(0.0, 1.0, 64) ** 2
     U = np.minimum(((yt - 0.22) / 0.028) ** (1.0 / 7.0), 1.0)
     a = 0.08 ** 2
-    prof = np.stack([yt, U, 0.0 * yt, a + 0 * yt, a + 0 * yt, a + 0 * yt, -0.3 * a + 0 * yt], axis=1)
+    prof = np.stack([yt, U, 0.0 * yt, a + 0 * yt, a + 0 * yt, a + 0 * yt, 0.0 * yt], axis=1)
     prim_inf = [props["rho_inf"], props["U_inf"], 0.0, 0.0, props["p_inf"]]
     rp = {k: float(v) for k, v in props.items() if k != "gas"}
     spec = {"name": f"ramp_{nx}x{ny}x{nz}", "blocks": ["ramp"], "ng": 3,
@@ -220,18 +221,20 @@ def ramp_spec(ns, nx=512, ny=128, nz=64, first_dy=1e-3, n_modes=64, seed=3, cfl=
     return spec, arrays, prim
 
 
-def ensemble_specs(n_cases=16, seed=2024, scheme=None, sgs="vreman"):
-    """`n_cases` channel cases: Re_b log-uniform in [5e3, 4e4], grids between
-    64x48x32 and 192x96x64 (multiples of 8), each split in x into 2-4 blocks of unequal
-    size.  Returns [(spec, arrays, props, seed)]."""
-    rng = np.random.default_rng(seed)
-    scheme = scheme or {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}}
+def ensemble_specs(n_cases=16, scheme=None, sgs="wale"):
+    """`n_cases` channel cases, case s drawn from `default_rng(s)`: Re_b log-uniform in
+    [5e3, 4e4], (nx, ny, nz) multiples of 16 in [64, 192] x [48, 96] x [32, 64], each
+    split in x into 2-4 blocks at random unequal cuts (every block at least 8 cells).
+    Numerics default to the two-block channel's: KEP(2), WALE.
+    Returns [(spec, arrays, props, seed)]."""
+    scheme = scheme or {"kind": "KEP", "order": 2}
     out = []
     for c in range(n_cases):
+        rng = np.random.default_rng(c)
         Re_b = float(np.exp(rng.uniform(np.log(5e3), np.log(4e4))))
-        nx = int(rng.integers(8, 25)) * 8
-        ny = int(rng.integers(6, 13)) * 8
-        nz = int(rng.integers(4, 9)) * 8
+        nx = int(rng.integers(4, 13)) * 16
+        ny = int(rng.integers(3, 7)) * 16
+        nz = int(rng.integers(2, 5)) * 16
         nb = int(rng.integers(2, 5))
         w = rng.uniform(0.5, 1.5, nb)
         cuts = np.concatenate([[0.0], np.cumsum(w) / w.sum()]) * nx

@@ -319,6 +319,39 @@ def test_fp32_bulk_channel_steps_close_to_fp64():
         assert abs(q_32 - q_64) / max(abs(q_64), 1e-30) < 1e-3
 
 
+def test_fp32_bulk_wale_where_the_float_expression_overflows():
+    """WALE in the FP32-bulk form at millimetre scale: channel at Re_b 5e3 (half height
+    0.7 mm) with 64 wall-normal cells.  The WALE numerator (Sd:Sd)^(3/2) of the initial
+    state exceeds the float range (checked on the host from finite differences of the
+    cell-centre velocities), yet the FP32-bulk RHS stays finite and within FP32
+    resolution of the FP64 RHS, and steps stay finite."""
+    spec, arrays = cases.channel_spec(1, 32, 64, 16, {"kind": "KEP", "order": 2}, "wale", Re_b=5e3)
+    pr = spec["props"]
+    X = arrays["nodes_B0"]
+    xc = 0.125 * sum(X[:, a:X.shape[1] - 1 + a, b:X.shape[2] - 1 + b, c:X.shape[3] - 1 + c]
+                     for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    vel = cases.channel_prim(xc, pr, 0)[1:4]
+    axes = (xc[0][:, 0, 0], xc[1][0, :, 0], xc[2][0, 0, :])
+    g = np.stack([np.stack([np.gradient(vel[i], axes[j], axis=j) for j in range(3)]) for i in range(3)])
+    g2 = np.einsum("ik...,kj...->ij...", g, g)
+    sd = 0.5 * (g2 + g2.transpose(1, 0, 2, 3, 4)) - np.eye(3).reshape(3, 3, 1, 1, 1) * np.trace(g2) / 3.0
+    assert float(((sd * sd).sum(axis=(0, 1)) ** 1.5).max()) > 10 * float(np.finfo(np.float32).max)
+    R = {}
+    for prec in ("fp64", "fp32"):
+        mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": prec})
+        Q = s.init_state(lambda xc: cases.channel_prim(xc, pr, 0))
+        R[prec] = host(s.rhs(Q))
+        if prec == "fp32":
+            for _ in range(3):
+                Q = s.advance(Q, s.max_dt(Q))
+            assert all(bool(torch.isfinite(q).all()) for q in Q)
+    blk = mesh.blocks[0]
+    assert np.isfinite(blk.interior(R["fp32"][0])).all()
+    err = golden_io.rel_err(blk.interior(R["fp32"][0]), blk.interior(R["fp64"][0]))
+    print("wale fp32 at mm scale", err)
+    assert err <= 2e-4
+
+
 def test_redundant_sweep_passes_dropped_only_when_exact():
     """The plan drops the second sweep pass for the channel (it reproduces pass 1 bit for
     bit) and keeps it where edge ghosts depend on it; golden ghost maps stay exact."""

@@ -363,10 +363,10 @@ WMX_HD R nut_smagorinsky(const R* g, R delta, R Cs) {
   return (cd * cd) * fsqrt<R>(R(2.0) * ss);
 }
 
-// reference sgs.py:21-30
+// S:S and Sd:Sd of the WALE model for the gradient g.
 template <class R>
-WMX_HD R nut_wale(const R* g, R delta, R Cw) {
-  R ss = R(0.0);
+WMX_HD void wale_invariants(const R* g, R& ss, R& sdsd) {
+  ss = R(0.0);
   R g2[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -377,7 +377,7 @@ WMX_HD R nut_wale(const R* g, R delta, R Cw) {
       g2[3 * i + j] = (g[3 * i + 0] * g[0 + j] + g[3 * i + 1] * g[3 + j]) + g[3 * i + 2] * g[6 + j];
     }
   R tr = (g2[0] + g2[4]) + g2[8];
-  R sdsd = R(0.0);
+  sdsd = R(0.0);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -385,10 +385,39 @@ WMX_HD R nut_wale(const R* g, R delta, R Cw) {
       R sd = R(0.5) * (g2[3 * i + j] + g2[3 * j + i]) - (i == j ? fdiv<R>(tr, R(3.0)) : R(0.0));
       sdsd = sdsd + sd * sd;
     }
+}
+
+// reference sgs.py:21-30.  The FP64 form evaluates the reference expression as written.
+// In float the numerator (Sd:Sd)^(3/2) grows like |g|^6 and overflows beyond
+// |g| ~ 2.6e6 1/s (near-wall shear of millimetre channels reaches 1e7), so the float
+// form evaluates the model on g/|g| (|g| the Frobenius norm) and scales back by |g|: the
+// model is homogeneous of degree one in g, the regulariser 1e-30 becomes 1e-30/|g|^5.
+template <class R>
+WMX_HD R nut_wale(const R* g, R delta, R Cw) {
   R cd = Cw * delta;
-  R num = (cd * cd) * (sdsd * fsqrt<R>(sdsd));
-  R den = ((ss * ss) * fsqrt<R>(ss) + fsqrt<R>(fsqrt<R>(sdsd)) * sdsd) + R(1e-30);
-  return fdiv<R>(num, den);
+  if constexpr (sizeof(R) == 4) {
+    R n2 = R(0.0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) n2 = n2 + g[k] * g[k];
+    if (!(n2 > R(0.0))) return R(0.0);
+    R gn = fsqrt<R>(n2);
+    R rg = fdiv<R>(R(1.0), gn);
+    R gs[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gs[k] = g[k] * rg;
+    R ss, sdsd;
+    wale_invariants<R>(gs, ss, sdsd);
+    R rg2 = rg * rg;
+    R num = (cd * cd) * gn * (sdsd * fsqrt<R>(sdsd));
+    R den = ((ss * ss) * fsqrt<R>(ss) + fsqrt<R>(fsqrt<R>(sdsd)) * sdsd) + R(1e-30) * ((rg2 * rg2) * rg);
+    return fdiv<R>(num, den);
+  } else {
+    R ss, sdsd;
+    wale_invariants<R>(g, ss, sdsd);
+    R num = (cd * cd) * (sdsd * fsqrt<R>(sdsd));
+    R den = ((ss * ss) * fsqrt<R>(ss) + fsqrt<R>(fsqrt<R>(sdsd)) * sdsd) + R(1e-30);
+    return fdiv<R>(num, den);
+  }
 }
 
 // reference sgs.py:33-42

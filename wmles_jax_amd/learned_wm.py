@@ -53,11 +53,13 @@ def _ptr(t):
 class LearnedWallModel:
     """Learned model with the Newton model `base` as its data source and fallback.
 
-    re_range / hd_range / tr_range: trained box of Re_h, h/Delta, T_h/T_w.
+    re_range / hd_range / tr_range: trained box of Re_h, h/Delta, T_h/T_w.  Points
+    outside the Re_h or T_h/T_w range take the Newton fallback; h/Delta is clamped into
+    its range (the equilibrium laws do not depend on it).
     n_polish: Newton steps from the network's u+.  tol: fallback when
     |F(u+)|/u+ > tol after the polish."""
 
-    def __init__(self, base=None, re_range=(1e2, 1e6), hd_range=(0.05, 4.0), tr_range=(0.5, 1.5),
+    def __init__(self, base=None, re_range=(1e2, 1e6), hd_range=(0.5, 4.0), tr_range=(0.8, 1.25),
                  n_polish=3, tol=1e-6, device=None):
         self.base = base or EquilibriumWallModel()
         self.kappa, self.B, self.n_iter = self.base.kappa, self.base.B, self.base.n_iter
