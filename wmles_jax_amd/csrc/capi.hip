@@ -105,7 +105,7 @@ struct wmx_plan {
   std::vector<InflowDev> inflows;
   // workspace
   size_t ws_bytes = 0;
-  std::vector<size_t> off_P, off_G, off_shell, off_sponge;
+  std::vector<size_t> off_P, off_G, off_shell, off_sponge, off_geo32;
   std::vector<std::vector<long long>> off_wall;   // per block, 6 faces (-1: none)
   size_t off_part = 0, off_scal = 0;
   char* ws = nullptr;
@@ -517,6 +517,11 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     B.case_id = d.case_id;
     if (d.case_id < 0 || d.case_id >= p->n_cases) { delete p; return fail("case_id out of range"); }
     B.cells = (long long)B.N[0] * B.N[1] * B.N[2];
+    // kernels address each plane / face-component array with 32-bit byte offsets
+    if ((long long)(B.N[0] + 1) * (B.N[1] + 1) * (B.N[2] + 1) * (long long)sizeof(double) >= (1LL << 32)) {
+      delete p;
+      return fail("block too large: every FP64 plane (cells or faces incl. ghosts) must stay below 4 GiB");
+    }
     B.vol = d.vol;
     B.xc = d.xc;
     for (int a = 0; a < 3; ++a) { B.S[a] = d.S[a]; B.xf[a] = d.xf[a]; }
@@ -605,6 +610,10 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     off = align_up(off + rec * NP * B.cells);
     p->off_G.push_back(off);
     off = align_up(off + rec * NGR * B.cells);
+    p->off_geo32.push_back(off);
+    if (nd.prec == WMX_PREC_FP32_BULK)
+      off = align_up(off + sizeof(float) * (3 * (size_t)(fcount(B.N, 0) + fcount(B.N, 1) + fcount(B.N, 2)) +
+                                            (size_t)B.cells));
     p->off_shell.push_back(off);
     off = align_up(off + sizeof(double) * 5 * B.shell_cells);
     p->off_sponge.push_back(off);
@@ -678,6 +687,17 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
     B.G = reinterpret_cast<double*>(p->ws + p->off_G[b]);
     B.Pf = reinterpret_cast<float*>(p->ws + p->off_P[b]);
     B.Gf = reinterpret_cast<float*>(p->ws + p->off_G[b]);
+    if (p->nd.prec == WMX_PREC_FP32_BULK) {
+      float* g = reinterpret_cast<float*>(p->ws + p->off_geo32[b]);
+      for (int d = 0; d < 3; ++d) {
+        B.Sf[d] = g;
+        g += 3 * fcount(B.N, d);
+      }
+      B.volf = g;
+    } else {
+      B.Sf[0] = B.Sf[1] = B.Sf[2] = nullptr;
+      B.volf = nullptr;
+    }
     B.shell = reinterpret_cast<double*>(p->ws + p->off_shell[b]);
     B.sponge_mean = reinterpret_cast<double*>(p->ws + p->off_sponge[b]);
     for (int f = 0; f < 6; ++f)
@@ -686,6 +706,17 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
   p->d_part = reinterpret_cast<double*>(p->ws + p->off_part);
   p->d_delta = reinterpret_cast<double*>(p->ws + p->off_scal);
   WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
+  if (p->nd.prec == WMX_PREC_FP32_BULK) {
+    // float geometry copies, made once: the geometry may still be in flight on the
+    // caller's stream, so the device is synchronised on both sides of the copies
+    WMX_CHECK(hipDeviceSynchronize());
+    for (int b = 0; b < p->nb; ++b) {
+      const BlockDev& B = p->hb[b];
+      for (int d = 0; d < 3; ++d) WMX_CHECK(launch_to_f32(B.S[d], B.Sf[d], 3 * fcount(B.N, d), nullptr));
+      WMX_CHECK(launch_to_f32(B.vol, B.volf, B.cells, nullptr));
+    }
+    WMX_CHECK(hipDeviceSynchronize());
+  }
   return 0;
 }
 

@@ -33,6 +33,8 @@ struct BlockDev {
   double* G;                // NGR planes (FP64 form)
   float* Pf;                // NP planes (FP32-bulk form; same workspace region as P)
   float* Gf;                // NGR planes (FP32-bulk form)
+  float* Sf[3];             // FP32-bulk form: float copies of S[d] (gradient kernel)
+  float* volf;              // FP32-bulk form: float copy of vol
   double* wall[6];          // WALL_PLANES planes of (n_t1, n_t2) per WallModel face
   double* sponge_mean;      // (5, n0, n1)
   double* shell;            // (5, shell cells) saved ghost values
@@ -97,6 +99,7 @@ hipError_t launch_fill_cells(const BlockDev* d_blks, const GhostOp* d_ops, int n
 hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng, InflowDev in,
                          Ptrs q, GasC gas, double time, const double* d_dt, double coef,
                          hipStream_t s);
+hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
@@ -156,6 +159,27 @@ template <class T>
 __device__ __forceinline__ gconst<T> gmem(const T* p) { return (gconst<T>)p; }
 template <class T>
 __device__ __forceinline__ gmut<T> gmem_w(T* p) { return (gmut<T>)p; }
+
+// Load / store at element `i` of an array whose base pointer is wave-uniform, with the
+// byte offset formed in 32 bits: the access compiles to the scalar-base global form
+// (64-bit base in SGPRs, one 32-bit offset VGPR).  The caller guarantees i * sizeof(T)
+// < 2^32 (plans reject blocks whose planes would exceed that).
+template <class T>
+__device__ __forceinline__ T ldu(const T* base, unsigned i) {
+  return *(gconst<T>)((const __attribute__((address_space(1))) char*)base + i * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stu(T* base, unsigned i, T v) {
+  *(gmut<T>)((__attribute__((address_space(1))) char*)base + i * (unsigned)sizeof(T)) = v;
+}
+
+// Workgroup w of a launch runs on XCD w % 8.  xcd_span maps it to a logical index so
+// that each XCD takes one contiguous span of the launch (sizes differ by at most one):
+// workgroups that share halo data then share an L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_span(int w, int n) {
+  const int q = n >> 3, r = n & 7, x = w & 7;
+  return x * q + min(x, r) + (w >> 3);
+}
 
 // tangential axes of a face normal to `axis`, increasing order
 WMX_HD void tangents(int axis, int* t1, int* t2) {

@@ -47,6 +47,31 @@ __device__ __forceinline__ ColT<R> load_col(PT P, long long cells, long long c) 
   return o;
 }
 
+// the same with wave-uniform plane bases and 32-bit element offsets
+template <class R>
+__device__ __forceinline__ ColT<R> load_col_u(const R* P, unsigned cells, unsigned c) {
+  ColT<R> o;
+  o.w.r = ldu(P, c);
+  o.w.u = ldu(P + cells, c);
+  o.w.v = ldu(P + 2 * cells, c);
+  o.w.w = ldu(P + 3 * cells, c);
+  o.w.p = ldu(P + 4 * cells, c);
+  o.w.e = ldu(P + P_E * cells, c);
+  o.T = ldu(P + P_T * cells, c);
+  return o;
+}
+
+template <class R>
+__device__ __forceinline__ ColGT<R> load_colg_u(const R* G, unsigned cells, unsigned c) {
+  ColGT<R> o;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) o.g.g[i] = ldu(G + i * cells, c);
+  o.g.mu = ldu(G + G_MU * cells, c);
+  o.g.k = ldu(G + G_K * cells, c);
+  o.th = ldu(G + G_TH * cells, c);
+  return o;
+}
+
 template <class R, class PT>
 __device__ __forceinline__ ColGT<R> load_colg(PT G, long long cells, long long c) {
   ColGT<R> o;
@@ -244,12 +269,14 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   const int jl = min(j0 + C2J, n1) - j0;
   const int kl = min(k0 + TK, n2) - k0;
   const int J = ng + j, K = ng + k;
-  const long long cells = B.cells;
-  const long long sI = (long long)N[1] * N[2];
+  // 32-bit element offsets from wave-uniform plane bases (every plane < 4 GiB)
+  const unsigned cells = (unsigned)B.cells;
+  const unsigned sI = (unsigned)N[1] * N[2];
   const double dt = d_dt ? d_dt[B.case_id] : dt_host;
-  const gconst<R> P = gmem(sizeof(R) == 8 ? (const R*)B.P : (const R*)B.Pf);
-  const gconst<R> G = gmem(sizeof(R) == 8 ? (const R*)B.G : (const R*)B.Gf);
-  const gconst<double> XC = gmem(B.xc);
+  static_assert(sizeof(R) == 4, "k_flux2 is the FP32-bulk kernel");
+  const R* P = (const R*)B.Pf;
+  const R* G = (const R*)B.Gf;
+  const double* XC = B.xc;
   // boxes the staged data may come from: primitives [0, N), gradients [ng-1, ng+n]
   auto p_ok = [&](int jj, int kk) { return jj >= 0 && jj < N[1] && kk >= 0 && kk < N[2]; };
   auto g_ok = [&](int jj, int kk) {
@@ -262,42 +289,42 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   ColGT<R> Gw[2];
   double Xw[2][3];
   const int I0 = ng + i0;
-  const long long cb = (long long)min(J, N[1] - 1) * N[2] + min(K, N[2] - 1);
+  const unsigned cb = (unsigned)min(J, N[1] - 1) * N[2] + min(K, N[2] - 1);
 #pragma unroll
-  for (int s = 1; s < 4; ++s) Pw[s] = load_col<R>(P, cells, (I0 - 3 + s) * sI + cb);
-  Gw[1] = load_colg<R>(G, cells, (I0 - 1) * sI + cb);
+  for (int s = 1; s < 4; ++s) Pw[s] = load_col_u<R>(P, cells, (unsigned)(I0 - 3 + s) * sI + cb);
+  Gw[1] = load_colg_u<R>(G, cells, (unsigned)(I0 - 1) * sI + cb);
 #pragma unroll
-  for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I0 - 1) * sI + cb];
+  for (int x = 0; x < 3; ++x) Xw[1][x] = ldu(XC + x * cells, (unsigned)(I0 - 1) * sI + cb);
   const int own_s = tj * TK + tk;               // this column's i-face slot (own threads)
   int lo = 0;                                    // slot of the current lower i face
   // halo ring job of this thread (plane-independent): 140 primitive slots (halo 2),
   // 66 gradient/centroid slots (halo 1); prefetched one plane ahead into registers
   constexpr int NRP = NP2 - C2J * TK, NRG = NG2 - C2J * TK;
   int ring = 0, ring_s = 0;                      // 1: primitive slot, 2: gradient slot
-  long long ring_off = 0;
+  unsigned ring_off = 0;
   if (tid < NRP) {
     int sj, sk;
     ring_slot(tid, 2, C2J, &sj, &sk);
     const int jj = ng + j0 - 2 + sj, kk = ng + k0 - 2 + sk;
-    if (p_ok(jj, kk)) { ring = 1; ring_s = sj * H2K + sk; ring_off = (long long)jj * N[2] + kk; }
+    if (p_ok(jj, kk)) { ring = 1; ring_s = sj * H2K + sk; ring_off = (unsigned)jj * N[2] + kk; }
   } else if (tid < NRP + NRG) {
     int sj, sk;
     ring_slot(tid - NRP, 1, C2J, &sj, &sk);
     const int jj = ng + j0 - 1 + sj, kk = ng + k0 - 1 + sk;
-    if (g_ok(jj, kk)) { ring = 2; ring_s = sj * G2K + sk; ring_off = (long long)jj * N[2] + kk; }
+    if (g_ok(jj, kk)) { ring = 2; ring_s = sj * G2K + sk; ring_off = (unsigned)jj * N[2] + kk; }
   }
   R rv[NGW];
   double rx[3];
   auto ring_load = [&](int Ip) {
-    const long long c = (long long)Ip * sI + ring_off;
+    const unsigned c = (unsigned)Ip * sI + ring_off;
     if (ring == 1) {
 #pragma unroll
-      for (int i = 0; i < NPW; ++i) rv[i] = P[(i < 5 ? i : (i == 5 ? P_E : P_T)) * cells + c];
+      for (int i = 0; i < NPW; ++i) rv[i] = ldu(P + (i < 5 ? i : (i == 5 ? P_E : P_T)) * cells, c);
     } else if (ring == 2) {
 #pragma unroll
-      for (int i = 0; i < NGW; ++i) rv[i] = G[i * cells + c];
+      for (int i = 0; i < NGW; ++i) rv[i] = ldu(G + i * cells, c);
 #pragma unroll
-      for (int x = 0; x < 3; ++x) rx[x] = XC[x * cells + c];
+      for (int x = 0; x < 3; ++x) rx[x] = ldu(XC + x * cells, c);
     }
   };
   for (int I = I0 - 1; I < ng + i1; ++I) {
@@ -311,7 +338,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     //      HBM (each group of loads is issued as a batch before its LDS stores)
     if (!first) {
       const int sp = (tj + 2) * H2K + (tk + 2), sg = (tj + 1) * G2K + (tk + 1);
-      const long long pl = (long long)I * sI;
+      const unsigned pl = (unsigned)I * sI;
       if (act) {
         sP[0][sp] = Pw[1].w.r; sP[1][sp] = Pw[1].w.u; sP[2][sp] = Pw[1].w.v; sP[3][sp] = Pw[1].w.w;
         sP[4][sp] = Pw[1].w.p; sP[5][sp] = Pw[1].w.e; sP[6][sp] = Pw[1].T;
@@ -321,11 +348,11 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
 #pragma unroll
         for (int x = 0; x < 3; ++x) sX[x][sg] = Xw[0][x];
       } else if (own) {                              // idle column of a partial tile
-        const long long c = pl + (long long)J * N[2] + K;
+        const unsigned c = pl + (unsigned)J * N[2] + K;
         if (p_ok(J, K)) {
           R vp[NPW];
 #pragma unroll
-          for (int i = 0; i < NPW; ++i) vp[i] = P[(i < 5 ? i : (i == 5 ? P_E : P_T)) * cells + c];
+          for (int i = 0; i < NPW; ++i) vp[i] = ldu(P + (i < 5 ? i : (i == 5 ? P_E : P_T)) * cells, c);
 #pragma unroll
           for (int i = 0; i < NPW; ++i) sP[i][sp] = vp[i];
         }
@@ -333,9 +360,9 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           R vg[NGW];
           double vx[3];
 #pragma unroll
-          for (int i = 0; i < NGW; ++i) vg[i] = G[i * cells + c];
+          for (int i = 0; i < NGW; ++i) vg[i] = ldu(G + i * cells, c);
 #pragma unroll
-          for (int x = 0; x < 3; ++x) vx[x] = XC[x * cells + c];
+          for (int x = 0; x < 3; ++x) vx[x] = ldu(XC + x * cells, c);
 #pragma unroll
           for (int i = 0; i < NGW; ++i) sG[i][sg] = vg[i];
 #pragma unroll
@@ -356,10 +383,10 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     // ---- load plane I+2 (primitives) and I+1 (gradients, centroids) into the window;
     //      issued after the halo loads (completion counts in issue order), needed by
     //      the last face task
-    Pw[3] = load_col<R>(P, cells, (I + 2) * sI + cb);
-    Gw[1] = load_colg<R>(G, cells, (I + 1) * sI + cb);
+    Pw[3] = load_col_u<R>(P, cells, (unsigned)(I + 2) * sI + cb);
+    Gw[1] = load_colg_u<R>(G, cells, (unsigned)(I + 1) * sI + cb);
 #pragma unroll
-    for (int x = 0; x < 3; ++x) Xw[1][x] = XC[x * cells + (I + 1) * sI + cb];
+    for (int x = 0; x < 3; ++x) Xw[1][x] = ldu(XC + x * cells, (unsigned)(I + 1) * sI + cb);
     __syncthreads();
     // ---- face fluxes: 0 j face below row tj, 1 k face left of column tk (rows tj < 15)
     //      or the upper edge k face of row tk (tj = 15), 2 i face I+1 (registers)
@@ -426,16 +453,16 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           int t1, t2;
           tangents(d, &t1, &t2);
           const int cc[3] = {fI, fJ, fK};
-          const long long nw = (long long)B.n[t1] * B.n[t2];
-          const long long w = (long long)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
-          const gconst<double> W = gmem(B.wall[2 * d + side]);
+          const unsigned nw = (unsigned)B.n[t1] * B.n[t2];
+          const unsigned w = (unsigned)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
+          const double* W = B.wall[2 * d + side];
 #pragma unroll
-          for (int v = 0; v < 5; ++v) F[v] = W[v * nw + w];
+          for (int v = 0; v < 5; ++v) F[v] = ldu(W + v * nw, w);
         } else {
-          const long long fi = fidx(N, d, fI, fJ, fK), fc = fcount(N, d);
-          const gconst<double> S = gmem(B.S[d]);
-          flux_core<R>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, S[fi], S[fc + fi],
-                       S[2 * fc + fi], F);
+          const unsigned fi = (unsigned)fidx(N, d, fI, fJ, fK), fc = (unsigned)fcount(N, d);
+          const double* S = B.S[d];
+          flux_core<R>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
+                       ldu(S + fc, fi), ldu(S + 2 * fc, fi), F);
         }
         if (task == 2) {
           const int hs = first ? 0 : lo ^ 1;
@@ -455,13 +482,13 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
       continue;
     }
     // the epilogue's state and volume loads go out before the barrier
-    const long long c = (long long)I * sI + (long long)J * N[2] + K;
-    const gconst<double> qi = gmem(qin.p[t.x]);
+    const unsigned c = (unsigned)I * sI + (unsigned)J * N[2] + K;
+    const double* qi = qin.p[t.x];
     double vq[5], vz[5], ivol = 0.0;
     if (act) {
-      ivol = gmem(B.vol)[c];
+      ivol = ldu(B.vol, c);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) vq[v] = qi[v * cells + c];
+      for (int v = 0; v < 5; ++v) vq[v] = ldu(qi + v * cells, c);
     }
     __syncthreads();
     if (act) {
@@ -475,7 +502,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         r[v] = a * ivol;
       }
       if (nd.n_sponge > 0) {
-        const double xcv = XC[c];
+        const double xcv = ldu(XC, c);
         const int i = I - ng;
         const long long mrow = (long long)i * n1 + j;
         const long long mplane = (long long)n0 * n1;
@@ -487,24 +514,24 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           const double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
 #pragma unroll
           for (int v = 0; v < 5; ++v)
-            r[v] = r[v] + (-sigma) * (vq[v] - gmem(B.sponge_mean)[v * mplane + mrow]);
+            r[v] = r[v] + (-sigma) * (vq[v] - ldu(B.sponge_mean + v * mplane, (unsigned)mrow));
         }
       }
-      const gmut<double> o = gmem_w(out.p[t.x]);
+      double* o = out.p[t.x];
       if (MODE == 0) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) o[v * cells + c] = r[v];
+        for (int v = 0; v < 5; ++v) stu(o + v * cells, c, r[v]);
       } else if (MODE == 1) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) o[v * cells + c] = vq[v] + dt * r[v];
+        for (int v = 0; v < 5; ++v) stu(o + v * cells, c, vq[v] + dt * r[v]);
       } else {
-        const gconst<double> qz = gmem(q0.p[t.x]);
+        const double* qz = q0.p[t.x];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) vz[v] = qz[v * cells + c];
+        for (int v = 0; v < 5; ++v) vz[v] = ldu(qz + v * cells, c);
         const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
         const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) o[v * cells + c] = a0 * vz[v] + a1 * (vq[v] + dt * r[v]);
+        for (int v = 0; v < 5; ++v) stu(o + v * cells, c, a0 * vz[v] + a1 * (vq[v] + dt * r[v]));
       }
     }
     // halo ring of the next plane, in flight while the next plane's staging waits.  No

@@ -36,50 +36,53 @@ __device__ double block_reduce(double v) {
   return r;
 }
 
-// interior cell l of block B -> ghosted linear index
-__device__ __forceinline__ long long interior_cell(const BlockDev& B, long long l, int* I, int* J,
-                                                   int* K) {
-  const int n1 = B.n[1], n2 = B.n[2], ng = B.ng;
-  *K = ng + (int)(l % n2);
-  *J = ng + (int)((l / n2) % n1);
-  *I = ng + (int)(l / ((long long)n1 * n2));
-  return cidx(B.N, *I, *J, *K);
+// interior cell l of block B -> ghosted linear index (32-bit: plans keep every plane
+// below 2^29 cells)
+__device__ __forceinline__ unsigned interior_cell(const BlockDev& B, unsigned l, int* I, int* J,
+                                                  int* K) {
+  const unsigned n1 = (unsigned)B.n[1], n2 = (unsigned)B.n[2], ng = (unsigned)B.ng;
+  const unsigned r = l / n2;
+  *K = (int)(ng + (l - r * n2));
+  *J = (int)(ng + r % n1);
+  *I = (int)(ng + r / n1);
+  return ((unsigned)*I * (unsigned)B.N[1] + (unsigned)*J) * (unsigned)B.N[2] + (unsigned)*K;
 }
 
 // Per-cell CFL time step, min over interiors; solver.py:139-150.
 __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__ blks,
                                                     const int* __restrict__ wg_prefix, int nblk,
                                                     Ptrs q, NumDev nd, double* __restrict__ part) {
-  const int wg = blockIdx.x;
+  const int wg = xcd_span(blockIdx.x, gridDim.x);
   const int b = find_block_r(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
-  const long long l = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  const long long nint = (long long)B.n[0] * B.n[1] * B.n[2];
+  const unsigned l = (unsigned)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const unsigned nint = (unsigned)B.n[0] * B.n[1] * B.n[2];
   double dtc = __builtin_huge_val();
   if (l < nint) {
     int I, J, K;
-    const long long c = interior_cell(B, l, &I, &J, &K);
-    const long long cells = B.cells;
+    const unsigned c = interior_cell(B, l, &I, &J, &K);
+    const unsigned cells = (unsigned)B.cells;
     const double* qb = q.p[b];
     const GasC& gas = nd.gas;
-    Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
-                          qb[4 * cells + c]);
+    Prim w = cons_to_prim(gas, ldu(qb, c), ldu(qb + cells, c), ldu(qb + 2 * cells, c),
+                          ldu(qb + 3 * cells, c), ldu(qb + 4 * cells, c));
     const double cs = sqrt(gas.gamma * gas.R * w.T);
     const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
     const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
-    const double vol = B.vol[c];
+    const double vol = ldu(B.vol, c);
     double lam = 0.0, lamv = 0.0;
     const int cc[3] = {I, J, K};
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double* S = B.S[d];
-      const long long fc = fcount(B.N, d);
+      const unsigned fc = (unsigned)fcount(B.N, d);
       int hi[3] = {cc[0], cc[1], cc[2]};
       hi[d] += 1;
-      const long long f0 = fidx(B.N, d, cc[0], cc[1], cc[2]);
-      const long long f1 = fidx(B.N, d, hi[0], hi[1], hi[2]);
-      const double s0 = 0.5 * (S[f0] + S[f1]);
-      const double s1 = 0.5 * (S[fc + f0] + S[fc + f1]);
-      const double s2 = 0.5 * (S[2 * fc + f0] + S[2 * fc + f1]);
+      const unsigned f0 = (unsigned)fidx(B.N, d, cc[0], cc[1], cc[2]);
+      const unsigned f1 = (unsigned)fidx(B.N, d, hi[0], hi[1], hi[2]);
+      const double s0 = 0.5 * (ldu(S, f0) + ldu(S, f1));
+      const double s1 = 0.5 * (ldu(S + fc, f0) + ldu(S + fc, f1));
+      const double s2 = 0.5 * (ldu(S + 2 * fc, f0) + ldu(S + 2 * fc, f1));
       const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
       lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
       lamv = lamv + (nu * (Sm * Sm)) / vol;
@@ -97,15 +100,15 @@ __global__ __launch_bounds__(256) void k_mf_partial(const BlockDev* __restrict__
                                                     double* __restrict__ part) {
   const int b = find_block_r(wg_prefix, nblk, blockIdx.x);
   const BlockDev& B = blks[b];
-  const long long l = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  const long long nint = (long long)B.n[0] * B.n[1] * B.n[2];
+  const unsigned l = (unsigned)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const unsigned nint = (unsigned)B.n[0] * B.n[1] * B.n[2];
   const int dir = case_dir[B.case_id];
   double num = 0.0, den = 0.0;
   if (l < nint && dir >= 0) {
     int I, J, K;
-    const long long c = interior_cell(B, l, &I, &J, &K);
-    const double vol = B.vol[c];
-    num = q.p[b][(1 + dir) * B.cells + c] * vol;
+    const unsigned c = interior_cell(B, l, &I, &J, &K);
+    const double vol = ldu(B.vol, c);
+    num = ldu(q.p[b] + (unsigned)(1 + dir) * (unsigned)B.cells, c) * vol;
     den = vol;
   }
   const double rn = block_reduce<false>(num);

@@ -148,127 +148,145 @@ __global__ __launch_bounds__(256) void k_grad_cell(const BlockDev* __restrict__ 
   G[G_TH * cells + c] = ducros<R>(g);
 }
 
-// FP32-bulk:// One thread per run of grad_run() consecutive i cells of one (j, k) column: the i
-// neighbours' phi and the shared i-face vectors stay in registers from one cell of the
-// run to the next, so the planes i-1 / i+1 (far apart in memory) are read about once
-// instead of three times.  Per cell the arithmetic is that of a single-cell evaluation.
-template <class R>
-__global__ __launch_bounds__(256) void k_grad_run(const BlockDev* __restrict__ blks,
-                                              const int* __restrict__ wg_prefix, int nblk,
-                                              NumDev nd) {
-  const int wg = blockIdx.x;
+// FP32-bulk gradient / SGS / Ducros.  One thread per run of grad_run(4) consecutive i
+// cells of one (j, k) column: the i neighbours' phi and the shared i-face vectors stay in
+// registers from one cell of the run to the next, so the planes i-1 / i+1 (far apart in
+// memory) are read about once instead of three times.  Per cell the arithmetic is that
+// of a single-cell evaluation.  Addressing: every plane / face-component array gets a
+// wave-uniform base pointer and each load a 32-bit byte offset from it (the plan checks
+// that every plane stays below 4 GiB), so loads take the scalar-base form and need one
+// 32-bit offset register instead of a 64-bit address pair each.  The SGS model is a
+// template parameter so that only one model is compiled into each instance.
+template <int SGS>
+__global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blks,
+                                                const int* __restrict__ wg_prefix, int nblk,
+                                                NumDev nd) {
+  const int wg = xcd_span(blockIdx.x, gridDim.x);
   const int b = find_block(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
   const int ng = B.ng;
   const int b0 = B.n[0] + 2, b1 = B.n[1] + 2, b2 = B.n[2] + 2;
-  constexpr int RUN = grad_run(sizeof(R));
+  constexpr int RUN = grad_run(sizeof(float));
   const int nrun = (b0 + RUN - 1) / RUN;
-  const long long l = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  if (l >= (long long)nrun * b1 * b2) return;
-  const int K = ng - 1 + (int)(l % b2);
-  const int J = ng - 1 + (int)((l / b2) % b1);
-  const int Ia = ng - 1 + (int)(l / ((long long)b1 * b2)) * RUN;
+  const int l = (wg - wg_prefix[b]) * (int)blockDim.x + (int)threadIdx.x;
+  if (l >= nrun * b1 * b2) return;
+  const int K = ng - 1 + l % b2;
+  const int J = ng - 1 + (l / b2) % b1;
+  const int Ia = ng - 1 + (l / (b1 * b2)) * RUN;
   const int Ib = min(Ia + RUN, ng + B.n[0] + 1);
-  const int* N = B.N;
-  const long long cells = B.cells;
-  const long long st[3] = {(long long)N[1] * N[2], (long long)N[2], 1};
-  const gconst<R> P = gmem(sizeof(R) == 8 ? (const R*)B.P : (const R*)B.Pf);
-  const gmut<R> G = gmem_w(sizeof(R) == 8 ? (R*)B.G : (R*)B.Gf);
-  const int pl[4] = {P_U, P_V, P_W, P_T};
-  const R h = R(0.5);
+  const int N1 = B.N[1], N2 = B.N[2];
+  const unsigned cells = (unsigned)B.cells;
+  const unsigned st0 = (unsigned)N1 * N2, st1 = (unsigned)N2;
+  // face-array strides: S[d] has N_d + 1 entries along d
+  const unsigned f0s0 = st0;                              // i faces: (I, J, K) -> I N1 N2 + J N2 + K
+  const unsigned f1s0 = (unsigned)(N1 + 1) * N2;          // j faces: M1 = N1 + 1
+  const unsigned f2s0 = (unsigned)N1 * (N2 + 1), f2s1 = (unsigned)(N2 + 1);
+  const unsigned fc0 = (unsigned)fcount(B.N, 0), fc1 = (unsigned)fcount(B.N, 1),
+                 fc2 = (unsigned)fcount(B.N, 2);
+  const float* Pf = B.Pf;
+  const float* pv[4] = {Pf + P_U * cells, Pf + P_V * cells, Pf + P_W * cells, Pf + P_T * cells};
+  // float copies of the face vectors and volumes (the arithmetic is FP32; same values)
+  const float* S0[3] = {B.Sf[0], B.Sf[0] + fc0, B.Sf[0] + 2 * (size_t)fc0};
+  const float* S1[3] = {B.Sf[1], B.Sf[1] + fc1, B.Sf[1] + 2 * (size_t)fc1};
+  const float* S2[3] = {B.Sf[2], B.Sf[2] + fc2, B.Sf[2] + 2 * (size_t)fc2};
+  float* Gf = B.Gf;
+  const float h = 0.5f;
   const GasC& gs = nd.gas;
-  // window: phi of planes I-1 and I, the i-face vector of face I
-  R phm[4], ph0[4], S0lo[3];
-  {
-    const long long c = cidx(N, Ia, J, K);
+  unsigned c = (unsigned)Ia * st0 + (unsigned)J * st1 + (unsigned)K;
+  unsigned f0 = c;                                        // i-face index of face (I, J, K)
+  unsigned f1 = (unsigned)Ia * f1s0 + (unsigned)J * st1 + (unsigned)K;
+  unsigned f2 = (unsigned)Ia * f2s0 + (unsigned)J * f2s1 + (unsigned)K;
+  float phm[4], ph0[4], S0lo[3];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      phm[v] = P[pl[v] * cells + c - st[0]];
-      ph0[v] = P[pl[v] * cells + c];
-    }
-    const gconst<double> S = gmem(B.S[0]);
-    const long long fc = fcount(N, 0), f = fidx(N, 0, Ia, J, K);
-#pragma unroll
-    for (int x = 0; x < 3; ++x) S0lo[x] = R(S[x * fc + f]);
+  for (int v = 0; v < 4; ++v) {
+    phm[v] = ldu(pv[v], c - st0);
+    ph0[v] = ldu(pv[v], c);
   }
+#pragma unroll
+  for (int x = 0; x < 3; ++x) S0lo[x] = ldu(S0[x], f0);
 #pragma unroll 1
   for (int I = Ia; I < Ib; ++I) {
-    const long long c = cidx(N, I, J, K);
-    R php[4], S0hi[3];
+    float php[4], S0hi[3];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) php[v] = P[pl[v] * cells + c + st[0]];
-    {
-      const gconst<double> S = gmem(B.S[0]);
-      const long long fc = fcount(N, 0), f = fidx(N, 0, I + 1, J, K);
+    for (int v = 0; v < 4; ++v) php[v] = ldu(pv[v], c + st0);
 #pragma unroll
-      for (int x = 0; x < 3; ++x) S0hi[x] = R(S[x * fc + f]);
+    for (int x = 0; x < 3; ++x) S0hi[x] = ldu(S0[x], f0 + f0s0);
+    float acc[12];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float flo_v = h * (phm[v] + ph0[v]);
+      const float fhi_v = h * (ph0[v] + php[v]);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) acc[3 * v + x] = fhi_v * S0hi[x] - flo_v * S0lo[x];
     }
-    R acc[12];
+    {
+      float Slo[3], Shi[3];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) acc[i] = R(0.0);
-    int cc[3] = {I, J, K};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      R Slo[3], Shi[3];
-      if (d == 0) {
-#pragma unroll
-        for (int x = 0; x < 3; ++x) { Slo[x] = S0lo[x]; Shi[x] = S0hi[x]; }
-      } else {
-        const gconst<double> S = gmem(B.S[d]);
-        const long long fc = fcount(N, d);
-        int hi[3] = {cc[0], cc[1], cc[2]};
-        hi[d] += 1;
-        const long long flo = fidx(N, d, cc[0], cc[1], cc[2]);
-        const long long fhi = fidx(N, d, hi[0], hi[1], hi[2]);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) { Slo[x] = R(S[x * fc + flo]); Shi[x] = R(S[x * fc + fhi]); }
+      for (int x = 0; x < 3; ++x) {
+        Slo[x] = ldu(S1[x], f1);
+        Shi[x] = ldu(S1[x], f1 + st1);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const R pm = d == 0 ? phm[v] : P[pl[v] * cells + c - st[d]];
-        const R pp = d == 0 ? php[v] : P[pl[v] * cells + c + st[d]];
-        const R flo_v = h * (pm + ph0[v]);
-        const R fhi_v = h * (ph0[v] + pp);
+        const float flo_v = h * (ldu(pv[v], c - st1) + ph0[v]);
+        const float fhi_v = h * (ph0[v] + ldu(pv[v], c + st1));
 #pragma unroll
         for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
       }
     }
-    const double vol = gmem(B.vol)[c];
-    const R volr = R(vol);
-    R g[12];
+    {
+      float Slo[3], Shi[3];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) g[i] = fdiv<R>(acc[i], volr);
-    const R rho = P[c];
-    const R mu = P[P_MU * cells + c];
-    R mut = R(0.0);
-    if (nd.sgs != WMX_SGS_NONE) {
-      // filter width vol^(1/3) (solver.py:82); FP32-bulk: single-precision cube root
-      R delta;
-      if constexpr (sizeof(R) == 4) delta = cbrtf((float)vol);
-      else delta = R(pow(vol, 1.0 / 3.0));
-      const R cs = R(nd.sgs_c);
-      R nut;
-      if (nd.sgs == WMX_SGS_SMAGORINSKY) nut = nut_smagorinsky<R>(g, delta, cs);
-      else if (nd.sgs == WMX_SGS_WALE) nut = nut_wale<R>(g, delta, cs);
-      else nut = nut_vreman<R>(g, delta, cs);
+      for (int x = 0; x < 3; ++x) {
+        Slo[x] = ldu(S2[x], f2);
+        Shi[x] = ldu(S2[x], f2 + 1);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float flo_v = h * (ldu(pv[v], c - 1) + ph0[v]);
+        const float fhi_v = h * (ph0[v] + ldu(pv[v], c + 1));
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
+      }
+    }
+    const float volr = ldu((const float*)B.volf, c);
+    float g[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) g[i] = fdiv<float>(acc[i], volr);
+    const float rho = ldu((const float*)Pf, c);
+    const float mu = ldu(Pf + P_MU * cells, c);
+    float mut;
+    if constexpr (SGS != WMX_SGS_NONE) {
+      // filter width vol^(1/3) (solver.py:82), single-precision cube root
+      const float delta = cbrtf(volr);
+      const float cs = (float)nd.sgs_c;
+      float nut;
+      if constexpr (SGS == WMX_SGS_SMAGORINSKY) nut = nut_smagorinsky<float>(g, delta, cs);
+      else if constexpr (SGS == WMX_SGS_WALE) nut = nut_wale<float>(g, delta, cs);
+      else nut = nut_vreman<float>(g, delta, cs);
       mut = rho * nut;
     } else {
-      mut = R(0.0) * rho;
+      mut = 0.0f * rho;
     }
 #pragma unroll
-    for (int i = 0; i < 12; ++i) G[i * cells + c] = g[i];
-    G[G_MU * cells + c] = mu + mut;
-    G[G_K * cells + c] = R(gs.cp) * (fdiv<R>(mu, R(gs.Pr)) + fdiv<R>(mut, R(gs.Pr_t)));
-    G[G_TH * cells + c] = ducros<R>(g);
+    for (int i = 0; i < 12; ++i) stu(Gf + i * cells, c, g[i]);
+    stu(Gf + G_MU * cells, c, mu + mut);
+    stu(Gf + G_K * cells, c, (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t)));
+    stu(Gf + G_TH * cells, c, ducros<float>(g));
 #pragma unroll
     for (int v = 0; v < 4; ++v) { phm[v] = ph0[v]; ph0[v] = php[v]; }
 #pragma unroll
     for (int x = 0; x < 3; ++x) S0lo[x] = S0hi[x];
+    c += st0;
+    f0 += f0s0;
+    f1 += f1s0;
+    f2 += f2s0;
   }
 }
-
-template __global__ void k_grad_cell<double>(const BlockDev*, const int*, int, NumDev);
-template __global__ void k_grad_run<float>(const BlockDev*, const int*, int, NumDev);
+template __global__ void k_grad32<WMX_SGS_NONE>(const BlockDev*, const int*, int, NumDev);
+template __global__ void k_grad32<WMX_SGS_SMAGORINSKY>(const BlockDev*, const int*, int, NumDev);
+template __global__ void k_grad32<WMX_SGS_WALE>(const BlockDev*, const int*, int, NumDev);
+template __global__ void k_grad32<WMX_SGS_VREMAN>(const BlockDev*, const int*, int, NumDev);
 
 // ------------------------------------------------------------------------------ wall
 // NET: the learned model (wave-cooperative network + Newton polish) replaces the
@@ -413,7 +431,21 @@ __global__ void k_wm_eval(NumDev nd, int mode, long long n, const double* __rest
   }
 }
 
+// float copy of a double array (FP32-bulk geometry copies)
+__global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
 // ----------------------------------------------------------------------- launchers
+hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long long nb = (n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)nb), dim3(256), 0, s, a, b, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_wm_eval(const NumDev& nd, int mode, long long n, const double* a, const double* b,
                           const double* c, const double* d, double T_wall, double* o0, double* o1,
                           hipStream_t s) {
@@ -436,9 +468,21 @@ hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int n
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
                        hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
-  if (nd.prec == WMX_PREC_FP32_BULK)
-    hipLaunchKernelGGL(k_grad_run<float>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
-  else
+  if (nd.prec == WMX_PREC_FP32_BULK) {
+    switch (nd.sgs) {
+      case WMX_SGS_NONE:
+        hipLaunchKernelGGL(k_grad32<WMX_SGS_NONE>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
+        break;
+      case WMX_SGS_SMAGORINSKY:
+        hipLaunchKernelGGL(k_grad32<WMX_SGS_SMAGORINSKY>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
+        break;
+      case WMX_SGS_WALE:
+        hipLaunchKernelGGL(k_grad32<WMX_SGS_WALE>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
+        break;
+      default:
+        hipLaunchKernelGGL(k_grad32<WMX_SGS_VREMAN>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
+    }
+  } else
     hipLaunchKernelGGL(k_grad_cell<double>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
   return hipGetLastError();
 }
