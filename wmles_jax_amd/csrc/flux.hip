@@ -21,6 +21,7 @@ constexpr int HJ = TJ + 4, HK = TK + 4;      // primitive region (halo 2)
 constexpr int GJ = TJ + 2, GK = TK + 2;      // gradient / centroid region (halo 1)
 constexpr int NPW = 7;                       // r u v w p e T
 constexpr int NGW = 15;                      // g[12], mu_eff, k_eff, theta
+constexpr int NFX = 6;                       // FP32-bulk face exchange: F'[5], w
 
 template <class R>
 struct ColT {                                // one cell's primitive record + T
@@ -83,14 +84,24 @@ __device__ __forceinline__ ColGT<R> load_colg(PT G, long long cells, long long c
   return o;
 }
 
-// Inviscid (scheme) + viscous flux through one face, both including the area, in the
-// bulk type R with the result in FP64.  For R = float the central part carries
-// p - p_ref and e_int - e_ref; their contribution p_ref S (momentum) and
-// e_ref m + p_ref u_bar.S (energy) is added back in FP64 with the FP64 face vector.
-// MUSCL-HLLC reconstructs the shifted pressure and adds p_ref before the Riemann
-// solve.  The cell-to-cell unit vector and distance come from FP64 centroids.
-// FP32-bulk face: blend, viscous flux and area products in FP32; the FP64 result is
-// the FP32 flux plus (1 - s) times the central part's gauge contribution.
+// FP32-bulk face flux in gauge form.  The records carry p' = p - p_ref and
+// e' = e_int - e_ref.  The full flux through face f is
+//   F_f = (1 - s) (Fc'_f + p_ref S_f [momentum] + e_ref m_f + p_ref u_f [energy]) + s Fu_f
+// (Fc' the central flux of the shifted records, m_f its mass flux, u_f its normal
+// velocity times the area, Fu the MUSCL-HLLC flux with the full pressure, s the hybrid
+// weight) plus the viscous flux.  Two identities move every large term out of the
+// per-face FP32 values:
+//  * sum over a cell's faces of p_ref S_f (outward) is p_ref times the closure of the
+//    FP64 geometry (zero to FP64 rounding), so (1 - s) p_ref S_f is replaced by
+//    -s p_ref S_f, which is non-zero only on upwind-blended faces;
+//  * (1 - s) m_f = M_f - s Fu_f[0], with M_f the blended mass flux the face already
+//    carries, so e_ref (1 - s) m_f sums to e_ref times the cell's mass divergence, which
+//    the FP64 epilogue forms from the same FP32 values.
+// The face therefore hands over F'_f = (1 - s) Fc'_f + s Fu_f (momentum - s p_ref S_f,
+// energy - s e_ref Fu_f[0]) + viscous, and w_f = (1 - s) u_f.  The epilogue adds
+// e_ref x (mass divergence) + p_ref x (sum of w_f) to the energy in FP64.  On faces with
+// s = 0 (central) every term keeps FP64 accuracy; on blended faces the error is that of
+// the FP32 HLLC flux itself.
 __device__ __forceinline__ float hybrid_weight(const NumDev& nd, float thL, float thR) {
   const float th = fmaxf(thL, thR);
   return fminf(fmaxf(fdiv<float>(th - float(nd.th0), float(nd.th1 - nd.th0)), 0.f), 1.f);
@@ -101,8 +112,7 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
                                               const CellWT<float>& R2, float thL, float thR,
                                               const CellGT<float>& gL, const CellGT<float>& gR,
                                               float TL, float TR, const double* xL, const double* xR,
-                                              double Sx, double Sy, double Sz, double* F) {
-  const float sx = float(Sx), sy = float(Sy), sz = float(Sz);
+                                              float sx, float sy, float sz, float* Ff, float* uw) {
   const int sch = nd.scheme;
   float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
   if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
@@ -112,31 +122,26 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
     if (keep) central_flux<true, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
     else central_flux<false, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
   }
+  const float wc = 1.f - s;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v];
   if (s > 0.f) {
     muscl_hllc_flux<float>(float(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz,
                            float(nd.p_ref), Fu);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) Fu[v] = s * Fu[v];
+    const float pr = float(nd.p_ref), er = float(nd.e_ref);
+    Ff[0] = Ff[0] + s * Fu[0];
+    Ff[1] = Ff[1] + s * (Fu[1] - pr * sx);
+    Ff[2] = Ff[2] + s * (Fu[2] - pr * sy);
+    Ff[3] = Ff[3] + s * (Fu[3] - pr * sz);
+    Ff[4] = Ff[4] + s * (Fu[4] - er * Fu[0]);
   }
-  const float wc = 1.f - s;
-  float Ff[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v] + Fu[v];
+  *uw = wc * un4;
   const float d0 = float(xR[0] - xL[0]), d1 = float(xR[1] - xL[1]), d2 = float(xR[2] - xL[2]);
   const float dist = fsqrt<float>((d0 * d0 + d1 * d1) + d2 * d2);
   const float id = fdiv<float>(1.f, dist);
   const float phL[4] = {L.u, L.v, L.w, TL};
   const float phR[4] = {Rc.u, Rc.v, Rc.w, TR};
   viscous_face_e<float>(gL, gR, phL, phR, d0 * id, d1 * id, d2 * id, dist, sx, sy, sz, Ff);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) F[v] = (double)Ff[v];
-  if (s < 1.f) {
-    const double w = (double)wc;
-    F[1] += w * (nd.p_ref * Sx);
-    F[2] += w * (nd.p_ref * Sy);
-    F[3] += w * (nd.p_ref * Sz);
-    F[4] += w * (nd.e_ref * (double)Fc[0] + nd.p_ref * (double)un4);
-  }
 }
 
 template <class R>
@@ -145,10 +150,7 @@ __device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2,
                                           const CellGT<R>& gL, const CellGT<R>& gR, R TL, R TR,
                                           const double* xL, const double* xR, double Sx, double Sy,
                                           double Sz, double* F) {
-  if constexpr (sizeof(R) == 4) {
-    flux_core_f32(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, Sx, Sy, Sz, F);
-    return;
-  }
+  static_assert(sizeof(R) == 8, "the FP32-bulk face flux is flux_core_f32");
   const R sx = R(Sx), sy = R(Sy), sz = R(Sz);
   const int sch = nd.scheme;
   const bool gauge = sizeof(R) == 4;
@@ -251,9 +253,10 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   __shared__ R sP[NPW][NP2];
   __shared__ R sG[NGW][NG2];
   __shared__ double sX[3][NG2];
-  __shared__ double sFj[5][(C2J + 1) * TK];
-  __shared__ double sFk[5][C2J * (TK + 1)];
-  __shared__ double sFi[2][5][C2J * TK];        // own i faces: lower / upper, ping-pong
+  // face exchange: F'[5] and w (gauge form, FP32)
+  __shared__ float sFj[NFX][(C2J + 1) * TK];
+  __shared__ float sFk[NFX][C2J * (TK + 1)];
+  __shared__ float sFi[2][NFX][C2J * TK];        // own i faces: lower / upper, ping-pong
   const int4 t = tiles[blockIdx.x];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           for (int x = 0; x < 3; ++x) { xL[x] = sX[x][gR_ - gst]; xR[x] = sX[x][gR_]; }
         }
       }
-      double F[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      float F[NFX] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (doit) {
         const int cf = d == 0 ? fI : (d == 1 ? fJ : fK);
         int side = -1;
@@ -455,25 +458,25 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           const int cc[3] = {fI, fJ, fK};
           const unsigned nw = (unsigned)B.n[t1] * B.n[t2];
           const unsigned w = (unsigned)(cc[t1] - ng) * B.n[t2] + (cc[t2] - ng);
-          const double* W = B.wall[2 * d + side];
+          const double* W = B.wall[2 * d + side];   // gauge form (p - p_ref) S, no mass flux
 #pragma unroll
-          for (int v = 0; v < 5; ++v) F[v] = ldu(W + v * nw, w);
+          for (int v = 0; v < 5; ++v) F[v] = (float)ldu(W + v * nw, w);
         } else {
           const unsigned fi = (unsigned)fidx(N, d, fI, fJ, fK), fc = (unsigned)fcount(N, d);
-          const double* S = B.S[d];
-          flux_core<R>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
-                       ldu(S + fc, fi), ldu(S + 2 * fc, fi), F);
+          const float* S = B.Sf[d];
+          flux_core_f32(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
+                        ldu(S + fc, fi), ldu(S + 2 * fc, fi), F, &F[5]);
         }
         if (task == 2) {
           const int hs = first ? 0 : lo ^ 1;
 #pragma unroll
-          for (int v = 0; v < 5; ++v) sFi[hs][v][own_s] = F[v];
+          for (int v = 0; v < NFX; ++v) sFi[hs][v][own_s] = F[v];
         } else if (d == 1) {
 #pragma unroll
-          for (int v = 0; v < 5; ++v) sFj[v][rj * TK + rk] = F[v];
+          for (int v = 0; v < NFX; ++v) sFj[v][rj * TK + rk] = F[v];
         } else {
 #pragma unroll
-          for (int v = 0; v < 5; ++v) sFk[v][rj * (TK + 1) + rk] = F[v];
+          for (int v = 0; v < NFX; ++v) sFk[v][rj * (TK + 1) + rk] = F[v];
         }
       }
     }
@@ -493,14 +496,20 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     __syncthreads();
     if (act) {
       ivol = 1.0 / ivol;                           // (FP32-bulk form: one reciprocal)
+      // outward sums in FP64 of the FP32 face values; the energy gets the gauge terms
+      // e_ref x (mass divergence) + p_ref x (sum of w)
+      double a[NFX];
+#pragma unroll
+      for (int v = 0; v < NFX; ++v) {
+        double t = 0.0 - ((double)sFi[lo ^ 1][v][own_s] - (double)sFi[lo][v][own_s]);
+        t = t - ((double)sFj[v][(tj + 1) * TK + tk] - (double)sFj[v][tj * TK + tk]);
+        t = t - ((double)sFk[v][tj * (TK + 1) + tk + 1] - (double)sFk[v][tj * (TK + 1) + tk]);
+        a[v] = t;
+      }
+      a[4] = (a[4] + nd.e_ref * a[0]) + nd.p_ref * a[5];
       double r[5];
 #pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        double a = 0.0 - (sFi[lo ^ 1][v][own_s] - sFi[lo][v][own_s]);
-        a = a - (sFj[v][(tj + 1) * TK + tk] - sFj[v][tj * TK + tk]);
-        a = a - (sFk[v][tj * (TK + 1) + tk + 1] - sFk[v][tj * (TK + 1) + tk]);
-        r[v] = a * ivol;
-      }
+      for (int v = 0; v < 5; ++v) r[v] = a[v] * ivol;
       if (nd.n_sponge > 0) {
         const double xcv = ldu(XC, c);
         const int i = I - ng;

@@ -360,8 +360,11 @@ __global__ __launch_bounds__(256) void k_wall(const BlockDev* __restrict__ blks,
   double sg = s == 1 ? 1.0 : -1.0;
   double* W = B.wall[2 * a + s];
   const long long nw = (long long)B.n[t1] * B.n[t2];
+  // FP32-bulk plans take the wall flux in gauge form: (p - p_ref) instead of p (the
+  // p_ref S terms of a cell's faces telescope; no mass flux, so no energy gauge term)
+  const double pw = nd.prec == WMX_PREC_FP32_BULK ? wb.p - nd.p_ref : wb.p;
   W[idx] = sg * 0.0;
-  for (int x = 0; x < 3; ++x) W[(1 + x) * nw + idx] = sg * ((wb.p * no[x] + tau * (up[x] / U)) * Sm);
+  for (int x = 0; x < 3; ++x) W[(1 + x) * nw + idx] = sg * ((pw * no[x] + tau * (up[x] / U)) * Sm);
   W[4 * nw + idx] = sg * ((-qw) * Sm);
   W[5 * nw + idx] = tau;
   W[6 * nw + idx] = qw;
