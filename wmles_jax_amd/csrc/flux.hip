@@ -107,26 +107,42 @@ __device__ __forceinline__ float hybrid_weight(const NumDev& nd, float thL, floa
   return fminf(fmaxf(fdiv<float>(th - float(nd.th0), float(nd.th1 - nd.th0)), 0.f), 1.f);
 }
 
+// Scheme configurations compiled as their own kernel instances (the face flux then
+// carries no runtime dispatch); SC_ANY reads everything from NumDev.
+enum { SC_ANY = 0, SC_HYB_KEP4_ALBADA = 1, SC_KEP2 = 2, SC_KEP4 = 3 };
+
+__host__ __device__ inline int scheme_config(const NumDev& nd) {
+  if (nd.scheme == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEP && nd.order != 2 && nd.limiter == 0)
+    return SC_HYB_KEP4_ALBADA;
+  if (nd.scheme == WMX_SCHEME_KEP) return nd.order == 2 ? SC_KEP2 : SC_KEP4;
+  return SC_ANY;
+}
+
+template <int SC>
 __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
                                               const CellWT<float>& L, const CellWT<float>& Rc,
                                               const CellWT<float>& R2, float thL, float thR,
                                               const CellGT<float>& gL, const CellGT<float>& gR,
                                               float TL, float TR, const double* xL, const double* xR,
                                               float sx, float sy, float sz, float* Ff, float* uw) {
-  const int sch = nd.scheme;
+  const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
+                  : (SC == SC_KEP2 || SC == SC_KEP4) ? WMX_SCHEME_KEP : nd.scheme;
+  const int order = SC == SC_KEP2 ? 2 : ((SC == SC_HYB_KEP4_ALBADA || SC == SC_KEP4) ? 4 : nd.order);
+  const int limiter = SC == SC_HYB_KEP4_ALBADA ? 0 : nd.limiter;
   float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
   if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
   float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Fu[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
   if (s < 1.f) {
-    const bool keep = sch == WMX_SCHEME_KEEP || (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP);
-    if (keep) central_flux<true, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
-    else central_flux<false, float>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
+    const bool keep = SC == SC_ANY && (sch == WMX_SCHEME_KEEP ||
+                                       (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP));
+    if (keep) central_flux<true, float>(order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
+    else central_flux<false, float>(order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
   }
   const float wc = 1.f - s;
 #pragma unroll
   for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v];
   if (s > 0.f) {
-    muscl_hllc_flux<float>(float(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz,
+    muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz,
                            float(nd.p_ref), Fu);
     const float pr = float(nd.p_ref), er = float(nd.e_ref);
     Ff[0] = Ff[0] + s * Fu[0];
@@ -243,7 +259,7 @@ __device__ __forceinline__ CellGT<R> sg_g(const R (*sG)[NG2], int s) {
 }  // namespace
 
 // MODE 0: r = RHS; 1: q + dt r; 2: 3/4 q0 + 1/4 (q + dt r); 3: 1/3 q0 + 2/3 (q + dt r)
-template <class R, int MODE>
+template <class R, int MODE, int SC>
 __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ blks,
                                                const int4* __restrict__ tiles, int ci_len,
                                                NumDev nd, Ptrs qin, Ptrs q0, Ptrs out,
@@ -464,7 +480,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         } else {
           const unsigned fi = (unsigned)fidx(N, d, fI, fJ, fK), fc = (unsigned)fcount(N, d);
           const float* S = B.Sf[d];
-          flux_core_f32(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
+          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
                         ldu(S + fc, fi), ldu(S + 2 * fc, fi), F, &F[5]);
         }
         if (task == 2) {
@@ -773,13 +789,21 @@ hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, in
   if (ntiles <= 0) return hipSuccess;
   dim3 grid(ntiles), block(NT);
   if (nd.prec == WMX_PREC_FP32_BULK) {
-#define WMX_F2(M) hipLaunchKernelGGL((k_flux2<float, M>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt)
-    switch (mode) {
-      case 0: WMX_F2(0); break;
-      case 1: WMX_F2(1); break;
-      case 2: WMX_F2(2); break;
-      default: WMX_F2(3); break;
+#define WMX_F2(M, C) hipLaunchKernelGGL((k_flux2<float, M, C>), grid, block, 0, s, d_blks, d_tiles, ci_len, nd, qin, q0, out, d_dt, dt)
+#define WMX_F2M(C)              \
+  switch (mode) {               \
+    case 0: WMX_F2(0, C); break; \
+    case 1: WMX_F2(1, C); break; \
+    case 2: WMX_F2(2, C); break; \
+    default: WMX_F2(3, C); break; \
+  }
+    switch (scheme_config(nd)) {
+      case SC_HYB_KEP4_ALBADA: WMX_F2M(SC_HYB_KEP4_ALBADA); break;
+      case SC_KEP2: WMX_F2M(SC_KEP2); break;
+      case SC_KEP4: WMX_F2M(SC_KEP4); break;
+      default: WMX_F2M(SC_ANY); break;
     }
+#undef WMX_F2M
 #undef WMX_F2
     return hipGetLastError();
   }
