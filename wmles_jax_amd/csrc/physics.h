@@ -455,7 +455,10 @@ struct WallModelC {      // reference wall_model.py:15-17 + host-evaluated const
   int n_iter;
 };
 
-// Spalding u_tau by safeguarded fixed-count Newton, reference wall_model.py:19-40.
+// Spalding u_tau by safeguarded fixed-count Newton, reference wall_model.py:19-40.  The
+// iteration map depends on up alone, so once an update returns its input exactly the
+// remaining iterations would repeat it: stopping there gives the fixed-count result bit
+// for bit.
 WMX_HD double spalding_utau(const WallModelC& m, double h, double U, double rho_w, double mu_w) {
   double k = m.kappa, ekb = m.ekb;
   double Reh = ((h * U) * rho_w) / mu_w;
@@ -468,7 +471,9 @@ WMX_HD double spalding_utau(const WallModelC& m, double h, double U, double rho_
     double F = (up + E) - Reh / up;
     double dF = (1.0 + (ekb * k) * poly2) + Reh / (up * up);
     double nw = up - F / dF;
-    up = fmin(fmax(nw, 0.5 * up), 100.0);
+    const double upn = fmin(fmax(nw, 0.5 * up), 100.0);
+    if (upn == up) break;
+    up = upn;
   }
   return U / up;
 }

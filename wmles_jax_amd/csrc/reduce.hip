@@ -178,11 +178,9 @@ __global__ __launch_bounds__(256) void k_mf_apply(const BlockDev* __restrict__ b
   const double qm = qb[(1 + dir) * cells + c];
   const double q0 = qb[c];
   const double dE = ((2.0 * qm) * dl + dl * dl) / (2.0 * q0);
-  qb[c] = q0 + 0.0;
-  for (int x = 0; x < 3; ++x) {
-    const double v = qb[(1 + x) * cells + c];
-    qb[(1 + x) * cells + c] = v + (x == dir ? 0.0 + dl : 0.0);
-  }
+  // only rho u_dir and rho E change (the reference adds 0 to the other components,
+  // which can only turn a -0.0 into +0.0)
+  qb[(1 + dir) * cells + c] = qm + (0.0 + dl);
   qb[4 * cells + c] = qb[4 * cells + c] + dE;
 }
 

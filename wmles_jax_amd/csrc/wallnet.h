@@ -129,7 +129,8 @@ __device__ __forceinline__ void net_eval(const unsigned short* __restrict__ sW,
 }
 
 // Spalding u+ (= U/u_tau) by the reference's safeguarded Newton from a given start
-// (wall_model.py:19-40; the reference starts from clip(sqrt(Re_h), 1, 25)).
+// (wall_model.py:19-40; the reference starts from clip(sqrt(Re_h), 1, 25)).  Stops at an
+// exact fixed point of the update, which the remaining iterations would only repeat.
 __device__ __forceinline__ double spalding_newton(const WallModelC& m, double Reh, double up, int iters) {
   const double k = m.kappa, ekb = m.ekb;
   for (int it = 0; it < iters; ++it) {
@@ -140,7 +141,9 @@ __device__ __forceinline__ double spalding_newton(const WallModelC& m, double Re
     const double F = (up + E) - Reh / up;
     const double dF = (1.0 + (ekb * k) * poly2) + Reh / (up * up);
     const double nw = up - F / dF;
-    up = fmin(fmax(nw, 0.5 * up), 100.0);
+    const double upn = fmin(fmax(nw, 0.5 * up), 100.0);
+    if (upn == up) break;
+    up = upn;
   }
   return up;
 }
