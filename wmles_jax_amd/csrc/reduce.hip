@@ -64,13 +64,11 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
     const unsigned cells = (unsigned)B.cells;
     const double* qb = q.p[b];
     const GasC& gas = nd.gas;
-    Prim w = cons_to_prim(gas, ldu(qb, c), ldu(qb + cells, c), ldu(qb + 2 * cells, c),
-                          ldu(qb + 3 * cells, c), ldu(qb + 4 * cells, c));
-    const double cs = sqrt(gas.gamma * gas.R * w.T);
-    const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
-    const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
+    // every load of the cell first (one memory round trip), then the arithmetic
+    double qv[5], Sv[3][3][2];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qv[v] = ldu(qb + v * cells, c);
     const double vol = ldu(B.vol, c);
-    double lam = 0.0, lamv = 0.0;
     const int cc[3] = {I, J, K};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -80,9 +78,22 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
       hi[d] += 1;
       const unsigned f0 = (unsigned)fidx(B.N, d, cc[0], cc[1], cc[2]);
       const unsigned f1 = (unsigned)fidx(B.N, d, hi[0], hi[1], hi[2]);
-      const double s0 = 0.5 * (ldu(S, f0) + ldu(S, f1));
-      const double s1 = 0.5 * (ldu(S + fc, f0) + ldu(S + fc, f1));
-      const double s2 = 0.5 * (ldu(S + 2 * fc, f0) + ldu(S + 2 * fc, f1));
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        Sv[d][x][0] = ldu(S + x * fc, f0);
+        Sv[d][x][1] = ldu(S + x * fc, f1);
+      }
+    }
+    Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
+    const double cs = sqrt(gas.gamma * gas.R * w.T);
+    const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
+    const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
+    double lam = 0.0, lamv = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double s0 = 0.5 * (Sv[d][0][0] + Sv[d][0][1]);
+      const double s1 = 0.5 * (Sv[d][1][0] + Sv[d][1][1]);
+      const double s2 = 0.5 * (Sv[d][2][0] + Sv[d][2][1]);
       const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
       lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
       lamv = lamv + (nu * (Sm * Sm)) / vol;
