@@ -231,6 +231,10 @@ int wmx_mass_flux(wmx_plan* plan, double* const* q, void* stream);
 /* tau_w and q_w of every WallModel face, concatenated in wmx_plan_wall_faces
    order, each face (n_t1, n_t2).  Replaces Solver.wall_stats (wmles/solver.py:163-179). */
 int wmx_wall_stats(wmx_plan* plan, double* const* q, double* d_tau, double* d_qw, void* stream);
+/* Plane statistics of one block: adds, for every interior j plane, the (i, k) means of
+   rho, u, v, w, uu, vv, ww, uv, uw, vw, T, p to d_acc[12][n_j] (FP64, device).
+   Replaces the averaging of PlaneStats.accumulate (wmles/channel5200.py:139-157). */
+int wmx_plane_stats(wmx_plan* plan, double* const* q, int32_t block, double* d_acc, void* stream);
 
 /* Fill the ng ghost layers of one boundary (non-interface, non-inflow) face of one
    block in place; synchronises the stream.  The BC protocol's fill(q, blk, geom,

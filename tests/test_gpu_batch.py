@@ -1,9 +1,11 @@
 """Batching form on the device: graph-captured steps and merged-case ensembles must
-reproduce the per-case, launch-by-launch step bit for bit."""
+reproduce the per-case, launch-by-launch step bit for bit; the device plane statistics
+against the oracle."""
 import numpy as np
 import pytest
 
 import cases
+import oracle
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -74,3 +76,35 @@ def test_autotuner_picks_a_candidate(tmp_path, monkeypatch):
     B, _ = s.step(Q, graph=False)
     for a, b in zip(A, B):                       # tiling changes scheduling, not results
         assert torch.equal(a, b)
+
+
+def test_plane_stats_match_the_oracle():
+    """PlaneStats on the device (wmx_plane_stats) against the oracle's plane means over two
+    accumulated states of a channel block, the Reynolds stresses and the wall units."""
+    from oracle import stats as OS
+    spec, arrays = cases.channel_spec(1, 24, 20, 16, {"kind": "KEP", "order": 2}, "wale")
+    mesh, s = cases.build(wmx, spec, arrays)
+    pr = spec["props"]
+    Q = s.init_state(lambda xc: cases.channel_prim(xc, pr, 0))
+    st = wmx.PlaneStats(mesh, s)
+    blk = mesh.blocks[0]
+    ost = OS.PlaneStats(st.y)
+    gas = oracle.Gas()
+    for _ in range(2):
+        Q = s.advance(Q, s.max_dt(Q))
+        st.accumulate(Q)
+        q = blk.interior(Q[0].cpu().numpy())
+        tau = np.mean([np.mean(v[0]) for v in s.wall_stats(Q).values()])
+        ost.accumulate(q, gas, tau)
+    y, avg = st.result()
+    _, ref = ost.result()
+    for k in ref:
+        scale = max(np.abs(ref[k]).max(), np.abs(ost.sum[k]).max() / ost.n * 1e-3, 1e-300)
+        assert np.abs(avg[k] - ref[k]).max() <= 1e-12 * scale, k
+    assert abs(st.mean_tau_w - ost.mean_tau_w) <= 1e-14 * abs(ost.mean_tau_w)
+    rho_w, mu_w = pr["rho0"], pr["mu0"]
+    ut, out = st.wall_units(st.mean_tau_w, rho_w, mu_w, pr["d"])
+    ut_o, out_o = ost.wall_units(ost.mean_tau_w, rho_w, mu_w, pr["d"])
+    assert abs(ut - ut_o) <= 1e-14 * ut_o
+    for k in out_o:
+        np.testing.assert_allclose(out[k], out_o[k], rtol=1e-10, atol=1e-12 * np.abs(out_o[k]).max())

@@ -94,6 +94,7 @@ def _load():
         "wmx_plan_tiling": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "wmx_mass_flux": ([P, PP, P], C.c_int),
         "wmx_wall_stats": ([P, PP, P, P, P], C.c_int),
+        "wmx_plane_stats": ([P, PP, C.c_int32, P, P], C.c_int),
         "wmx_fill_face": ([C.POINTER(BlockDesc), C.c_int32, C.c_int32, C.POINTER(NumericsDesc), P,
                            C.c_double, P], C.c_int),
         "wmx_wall_model_eval": ([C.POINTER(NumericsDesc), C.c_int32, C.c_int64, P, P, P, P,

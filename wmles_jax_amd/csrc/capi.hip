@@ -107,7 +107,8 @@ struct wmx_plan {
   size_t ws_bytes = 0;
   std::vector<size_t> off_P, off_G, off_shell, off_sponge, off_geo32;
   std::vector<std::vector<long long>> off_wall;   // per block, 6 faces (-1: none)
-  size_t off_part = 0, off_scal = 0;
+  size_t off_part = 0, off_scal = 0, off_stat = 0;
+  double* d_stat = nullptr;           // plane-statistics partials
   char* ws = nullptr;
   double* d_part = nullptr;
   double* d_delta = nullptr;
@@ -630,6 +631,12 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   off = align_up(off + sizeof(double) * 2 * (size_t)std::max(1, nwg_of(p->pre_int)));
   p->off_scal = off;
   off = align_up(off + sizeof(double) * 64);
+  {
+    int n1max = 1;
+    for (const BlockDev& B : p->hb) n1max = std::max(n1max, B.n[1]);
+    p->off_stat = off;
+    off = align_up(off + sizeof(double) * (size_t)n1max * PLANE_CHUNKS * NSTAT);
+  }
   p->ws_bytes = off;
   // device metadata
   std::vector<int> cdir(p->case_dir.begin(), p->case_dir.end());
@@ -705,6 +712,7 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
   }
   p->d_part = reinterpret_cast<double*>(p->ws + p->off_part);
   p->d_delta = reinterpret_cast<double*>(p->ws + p->off_scal);
+  p->d_stat = reinterpret_cast<double*>(p->ws + p->off_stat);
   WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
   if (p->nd.prec == WMX_PREC_FP32_BULK) {
     // float geometry copies, made once: the geometry may still be in flight on the
@@ -1013,6 +1021,16 @@ int wmx_wall_stats(wmx_plan* p, double* const* q, double* d_tau, double* d_qw, v
                              hipMemcpyDeviceToDevice, s));
     off += w.count;
   }
+  return 0;
+}
+
+int wmx_plane_stats(wmx_plan* p, double* const* q, int32_t block, double* d_acc, void* stream) {
+  if (check_plan(p)) return 1;
+  if (!q || !d_acc) return fail("null argument");
+  if (block < 0 || block >= p->nb) return fail("plane stats: block out of range");
+  hipStream_t s = as_stream(stream);
+  WMX_CHECK(launch_plane_stats(p->d_blks, p->hb[block], block, make_ptrs(p, q), p->nd.gas, p->d_stat,
+                               d_acc, s));
   return 0;
 }
 

@@ -195,6 +195,58 @@ __global__ __launch_bounds__(256) void k_mf_apply(const BlockDev* __restrict__ b
   qb[4 * cells + c] = qb[4 * cells + c] + dE;
 }
 
+// Plane statistics of one block (wmles/channel5200.py:139-157): per interior j plane the
+// (i, k) means of rho, u, v, w, uu, vv, ww, uv, uw, vw, T, p.  Partial sums per
+// (plane, chunk) workgroup, then one thread per plane sums its chunks in order and adds
+// the means to the accumulator acc[NSTAT][n1].
+__global__ __launch_bounds__(256) void k_plane_partial(const BlockDev* __restrict__ blks, int blk,
+                                                       Ptrs q, GasC gas, int chunk,
+                                                       double* __restrict__ part) {
+  const BlockDev& B = blks[blk];
+  const int j = blockIdx.y, ch = blockIdx.x, nch = gridDim.x;
+  const int n0 = B.n[0], n2 = B.n[2], ng = B.ng;
+  const long long cells = B.cells;
+  const long long tot = (long long)n0 * n2;
+  const long long l0 = (long long)ch * chunk, l1 = l0 + chunk < tot ? l0 + chunk : tot;
+  const double* qb = q.p[blk];
+  double a[NSTAT];
+#pragma unroll
+  for (int v = 0; v < NSTAT; ++v) a[v] = 0.0;
+  for (long long l = l0 + threadIdx.x; l < l1; l += blockDim.x) {
+    const int i = (int)(l / n2), k = (int)(l % n2);
+    const long long c = cidx(B.N, ng + i, ng + j, ng + k);
+    const Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
+                                qb[4 * cells + c]);
+    a[0] += w.r;
+    a[1] += w.u;
+    a[2] += w.v;
+    a[3] += w.w;
+    a[4] += w.u * w.u;
+    a[5] += w.v * w.v;
+    a[6] += w.w * w.w;
+    a[7] += w.u * w.v;
+    a[8] += w.u * w.w;
+    a[9] += w.v * w.w;
+    a[10] += w.T;
+    a[11] += w.p;
+  }
+  for (int v = 0; v < NSTAT; ++v) {
+    const double r = block_reduce<false>(a[v]);
+    if (threadIdx.x == 0) part[((long long)j * nch + ch) * NSTAT + v] = r;
+  }
+}
+
+__global__ void k_plane_final(int n1, int nch, double inv_count, const double* __restrict__ part,
+                              double* __restrict__ acc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n1) return;
+  for (int v = 0; v < NSTAT; ++v) {
+    double s = 0.0;
+    for (int ch = 0; ch < nch; ++ch) s += part[((long long)j * nch + ch) * NSTAT + v];
+    acc[(long long)v * n1 + j] += s * inv_count;
+  }
+}
+
 // Volume-weighted spanwise (k) mean of the interior state per (i, j) row
 // (smooth_ramp.py:214-216), reduced in numpy's add.reduce order.  One thread per row.
 __global__ void k_sponge_mean(const BlockDev* __restrict__ blks, int blk, Ptrs q) {
@@ -220,6 +272,19 @@ hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk,
                              const NumDev& nd, double* d_part, hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_dt_partial, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_plane_stats(const BlockDev* d_blks, const BlockDev& hb, int blk, Ptrs q,
+                              const GasC& gas, double* d_part, double* d_acc, hipStream_t s) {
+  const long long tot = (long long)hb.n[0] * hb.n[2];
+  const int chunk = (int)((tot + PLANE_CHUNKS - 1) / PLANE_CHUNKS);
+  hipLaunchKernelGGL(k_plane_partial, dim3(PLANE_CHUNKS, hb.n[1]), dim3(256), 0, s, d_blks, blk, q, gas,
+                     chunk, d_part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_plane_final, dim3((hb.n[1] + 63) / 64), dim3(64), 0, s, hb.n[1], PLANE_CHUNKS,
+                     1.0 / (double)tot, d_part, d_acc);
   return hipGetLastError();
 }
 
