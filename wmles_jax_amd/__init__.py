@@ -20,10 +20,11 @@ from . import smooth_ramp
 from .smooth_ramp import SmoothRampInflow, SmoothRampSponge, flow_properties
 from .ensemble import EnsembleSolver
 from .stats import PlaneStats
+from . import io_vtk
 from . import autotune
 
 __all__ = ["HAS_JAX", "Gas", "Block", "Mesh", "Geom", "Interface", "FACE_NAMES", "sl", "BC",
            "SlipWall", "NoSlipWall", "Dirichlet", "CharacteristicFarfield", "Extrapolate",
            "WallModel", "EquilibriumWallModel", "LearnedWallModel", "KEP", "KEEP", "MUSCLHLLC", "Hybrid", "make_sgs",
            "Solver", "ConstantMassFlux", "smooth_ramp", "SmoothRampInflow", "SmoothRampSponge",
-           "flow_properties", "EnsembleSolver", "PlaneStats", "autotune"]
+           "flow_properties", "EnsembleSolver", "PlaneStats", "io_vtk", "autotune"]

@@ -6,6 +6,12 @@ fails to load raises immediately.  There is no CPU fallback.
 import ctypes as C
 import os
 
+# torch first: its HIP runtime (libamdhip64.so.7) must be the one in the process before
+# the library is loaded, so that the library binds to that same runtime.  Loading the
+# library first would bring in a second copy of the runtime from /opt/rocm, and two
+# runtimes in one process do not share the device.
+import torch  # noqa: F401
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("WMX_LIB", os.path.join(_HERE, "libwmles_amd.so"))
 
