@@ -21,10 +21,13 @@ from .smooth_ramp import SmoothRampInflow, SmoothRampSponge, flow_properties
 from .ensemble import EnsembleSolver
 from .stats import PlaneStats
 from . import io_vtk
+from .plot3d import read_plot3d, write_plot3d
+from .nmf import read_nmf, respan, extrude_2d, build_mesh, describe
 from . import autotune
 
 __all__ = ["HAS_JAX", "Gas", "Block", "Mesh", "Geom", "Interface", "FACE_NAMES", "sl", "BC",
            "SlipWall", "NoSlipWall", "Dirichlet", "CharacteristicFarfield", "Extrapolate",
            "WallModel", "EquilibriumWallModel", "LearnedWallModel", "KEP", "KEEP", "MUSCLHLLC", "Hybrid", "make_sgs",
            "Solver", "ConstantMassFlux", "smooth_ramp", "SmoothRampInflow", "SmoothRampSponge",
-           "flow_properties", "EnsembleSolver", "PlaneStats", "io_vtk", "autotune"]
+           "flow_properties", "EnsembleSolver", "PlaneStats", "io_vtk", "autotune", "read_plot3d", "write_plot3d",
+           "read_nmf", "respan", "extrude_2d", "build_mesh", "describe"]

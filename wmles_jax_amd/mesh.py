@@ -61,7 +61,7 @@ class Block:
     """Ghosted structured block (mesh.py:88-121)."""
 
     def __init__(self, name, nodes, ng=3):
-        nodes = np.asarray(nodes, float)
+        nodes = np.ascontiguousarray(nodes, dtype=float)    # views (transposes, flips) copied
         if nodes.ndim != 4 or nodes.shape[0] != 3:
             raise ValueError("nodes must be (3, ni+1, nj+1, nk+1)")
         if ng != 3:
