@@ -610,7 +610,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     p->off_P.push_back(off);
     off = align_up(off + rec * NP * B.cells);
     p->off_G.push_back(off);
-    off = align_up(off + rec * NGR * B.cells);
+    off = align_up(off + rec * (nd.prec == WMX_PREC_FP32_BULK ? 4 * G32_GROUPS : NGR) * B.cells);
     p->off_geo32.push_back(off);
     if (nd.prec == WMX_PREC_FP32_BULK)
       off = align_up(off + sizeof(float) * (3 * (size_t)(fcount(B.N, 0) + fcount(B.N, 1) + fcount(B.N, 2)) +

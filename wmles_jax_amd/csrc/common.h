@@ -14,6 +14,13 @@ constexpr int WALL_PLANES = 7;  // Fw[5], tau, qw
 
 enum PrimPlane { P_R = 0, P_U, P_V, P_W, P_P, P_T, P_MU, P_E };
 enum GradPlane { G_MU = 12, G_K = 13, G_TH = 14 };
+// FP64 records are plane-major (plane v of cell c at v * cells + c).  FP32-bulk records
+// move in 16-byte groups: the primitive record is two planes of float4,
+// Pf[4 c ..] = (u, v, w, T') and Pf[4 cells + 4 c ..] = (rho, p', mu, e'), so the
+// gradient kernel reads one group per neighbour; the gradient record is four planes of
+// float4, Gf[4 (q cells + c) ..] for q = 0..3 holding GradPlane entries 4q..4q+3 (the last
+// group (mu_eff, k_eff, theta, 0)).  Every 16-byte access of a wavefront is contiguous.
+constexpr int G32_GROUPS = 4;
 
 struct Ptrs {
   double* p[WMX_MAX_BLOCKS];
@@ -183,6 +190,23 @@ __device__ __forceinline__ void stu(T* base, unsigned i, T v) {
 __device__ __forceinline__ int xcd_span(int w, int n) {
   const int q = n >> 3, r = n & 7, x = w & 7;
   return x * q + min(x, r) + (w >> 3);
+}
+
+// 16- and 8-byte record accesses (global address space; rec 16- / 8-byte aligned: the
+// workspace is 256-byte aligned and the record strides are multiples of 4 floats)
+typedef float wmx_f4 __attribute__((ext_vector_type(4)));
+typedef float wmx_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 ld4(const float* rec) {
+  const wmx_f4 v = *(const __attribute__((address_space(1))) wmx_f4*)__builtin_assume_aligned(rec, 16);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ld2(const float* rec) {
+  const wmx_f2 v = *(const __attribute__((address_space(1))) wmx_f2*)__builtin_assume_aligned(rec, 8);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st4(float* rec, float4 v) {
+  wmx_f4 w = {v.x, v.y, v.z, v.w};
+  *(__attribute__((address_space(1))) wmx_f4*)__builtin_assume_aligned(rec, 16) = w;
 }
 
 // tangential axes of a face normal to `axis`, increasing order

@@ -48,28 +48,24 @@ __device__ __forceinline__ ColT<R> load_col(PT P, long long cells, long long c) 
   return o;
 }
 
-// the same with wave-uniform plane bases and 32-bit element offsets
-template <class R>
-__device__ __forceinline__ ColT<R> load_col_u(const R* P, unsigned cells, unsigned c) {
-  ColT<R> o;
-  o.w.r = ldu(P, c);
-  o.w.u = ldu(P + cells, c);
-  o.w.v = ldu(P + 2 * cells, c);
-  o.w.w = ldu(P + 3 * cells, c);
-  o.w.p = ldu(P + 4 * cells, c);
-  o.w.e = ldu(P + P_E * cells, c);
-  o.T = ldu(P + P_T * cells, c);
+// the same from the FP32-bulk records (two / four 16-byte loads)
+__device__ __forceinline__ ColT<float> load_col32(const float* P, size_t cells, size_t c) {
+  const float4 a = ld4(P + 4 * c), b = ld4(P + 4 * (cells + c));
+  ColT<float> o;
+  o.w.u = a.x; o.w.v = a.y; o.w.w = a.z; o.T = a.w;
+  o.w.r = b.x; o.w.p = b.y; o.w.e = b.w;
   return o;
 }
 
-template <class R>
-__device__ __forceinline__ ColGT<R> load_colg_u(const R* G, unsigned cells, unsigned c) {
-  ColGT<R> o;
-#pragma unroll
-  for (int i = 0; i < 12; ++i) o.g.g[i] = ldu(G + i * cells, c);
-  o.g.mu = ldu(G + G_MU * cells, c);
-  o.g.k = ldu(G + G_K * cells, c);
-  o.th = ldu(G + G_TH * cells, c);
+__device__ __forceinline__ ColGT<float> load_colg32(const float* G, size_t cells, size_t c) {
+  const float* r = G + 4 * c;
+  const size_t q = 4 * cells;
+  const float4 a = ld4(r), b = ld4(r + q), d = ld4(r + 2 * q), e = ld4(r + 3 * q);
+  ColGT<float> o;
+  o.g.g[0] = a.x; o.g.g[1] = a.y; o.g.g[2] = a.z; o.g.g[3] = a.w;
+  o.g.g[4] = b.x; o.g.g[5] = b.y; o.g.g[6] = b.z; o.g.g[7] = b.w;
+  o.g.g[8] = d.x; o.g.g[9] = d.y; o.g.g[10] = d.z; o.g.g[11] = d.w;
+  o.g.mu = e.x; o.g.k = e.y; o.th = e.z;
   return o;
 }
 
@@ -310,8 +306,8 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   const int I0 = ng + i0;
   const unsigned cb = (unsigned)min(J, N[1] - 1) * N[2] + min(K, N[2] - 1);
 #pragma unroll
-  for (int s = 1; s < 4; ++s) Pw[s] = load_col_u<R>(P, cells, (unsigned)(I0 - 3 + s) * sI + cb);
-  Gw[1] = load_colg_u<R>(G, cells, (unsigned)(I0 - 1) * sI + cb);
+  for (int s = 1; s < 4; ++s) Pw[s] = load_col32(P, cells, (size_t)(I0 - 3 + s) * sI + cb);
+  Gw[1] = load_colg32(G, cells, (size_t)(I0 - 1) * sI + cb);
 #pragma unroll
   for (int x = 0; x < 3; ++x) Xw[1][x] = ldu(XC + x * cells, (unsigned)(I0 - 1) * sI + cb);
   const int own_s = tj * TK + tk;               // this column's i-face slot (own threads)
@@ -337,15 +333,18 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   auto ring_load = [&](int Ip) {
     const unsigned c = (unsigned)Ip * sI + ring_off;
     if (ring == 1) {
-#pragma unroll
-      for (int i = 0; i < NPW; ++i) rv[i] = ldu(P + (i < 5 ? i : (i == 5 ? P_E : P_T)) * cells, c);
+      const ColT<float> o = load_col32(P, cells, c);
+      rv[0] = o.w.r; rv[1] = o.w.u; rv[2] = o.w.v; rv[3] = o.w.w; rv[4] = o.w.p; rv[5] = o.w.e; rv[6] = o.T;
     } else if (ring == 2) {
+      const ColGT<float> o = load_colg32(G, cells, c);
 #pragma unroll
-      for (int i = 0; i < NGW; ++i) rv[i] = ldu(G + i * cells, c);
+      for (int i = 0; i < 12; ++i) rv[i] = o.g.g[i];
+      rv[12] = o.g.mu; rv[13] = o.g.k; rv[14] = o.th;
 #pragma unroll
       for (int x = 0; x < 3; ++x) rx[x] = ldu(XC + x * cells, c);
     }
   };
+
   for (int I = I0 - 1; I < ng + i1; ++I) {
     const bool first = I == I0 - 1;
     // ---- shift the window (planes I-1..I+1 of primitives, I of gradients/centroids)
@@ -369,17 +368,18 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
       } else if (own) {                              // idle column of a partial tile
         const unsigned c = pl + (unsigned)J * N[2] + K;
         if (p_ok(J, K)) {
-          R vp[NPW];
-#pragma unroll
-          for (int i = 0; i < NPW; ++i) vp[i] = ldu(P + (i < 5 ? i : (i == 5 ? P_E : P_T)) * cells, c);
+          const ColT<float> o = load_col32(P, cells, c);
+          const R vp[NPW] = {o.w.r, o.w.u, o.w.v, o.w.w, o.w.p, o.w.e, o.T};
 #pragma unroll
           for (int i = 0; i < NPW; ++i) sP[i][sp] = vp[i];
         }
         if (g_ok(J, K)) {
+          const ColGT<float> og = load_colg32(G, cells, c);
           R vg[NGW];
           double vx[3];
 #pragma unroll
-          for (int i = 0; i < NGW; ++i) vg[i] = ldu(G + i * cells, c);
+          for (int i = 0; i < 12; ++i) vg[i] = og.g.g[i];
+          vg[12] = og.g.mu; vg[13] = og.g.k; vg[14] = og.th;
 #pragma unroll
           for (int x = 0; x < 3; ++x) vx[x] = ldu(XC + x * cells, c);
 #pragma unroll
@@ -402,8 +402,8 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     // ---- load plane I+2 (primitives) and I+1 (gradients, centroids) into the window;
     //      issued after the halo loads (completion counts in issue order), needed by
     //      the last face task
-    Pw[3] = load_col_u<R>(P, cells, (unsigned)(I + 2) * sI + cb);
-    Gw[1] = load_colg_u<R>(G, cells, (unsigned)(I + 1) * sI + cb);
+    Pw[3] = load_col32(P, cells, (size_t)(I + 2) * sI + cb);
+    Gw[1] = load_colg32(G, cells, (size_t)(I + 1) * sI + cb);
 #pragma unroll
     for (int x = 0; x < 3; ++x) Xw[1][x] = ldu(XC + x * cells, (unsigned)(I + 1) * sI + cb);
     __syncthreads();

@@ -56,15 +56,10 @@ __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blk
   const double* qb = q.p[b];
   Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
                         qb[4 * cells + c]);
-  float* P = B.Pf;
-  P[c] = (float)w.r;
-  P[cells + c] = (float)w.u;
-  P[2 * cells + c] = (float)w.v;
-  P[3 * cells + c] = (float)w.w;
-  P[4 * cells + c] = (float)(w.p - nd.p_ref);
-  P[5 * cells + c] = (float)(w.T - nd.T_gauge);
-  P[6 * cells + c] = (float)mu_suth(gas, w.T);
-  P[7 * cells + c] = (float)(w.p / ((gas.gamma - 1.0) * w.r) - nd.e_ref);
+  st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - nd.T_gauge)));
+  st4(B.Pf + 4 * ((size_t)cells + c),
+      make_float4((float)w.r, (float)(w.p - nd.p_ref), (float)mu_suth(gas, w.T),
+                  (float)(w.p / ((gas.gamma - 1.0) * w.r) - nd.e_ref)));
 }
 
 // ------------------------------------------------------------------------------ grad
@@ -184,7 +179,11 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
   const unsigned fc0 = (unsigned)fcount(B.N, 0), fc1 = (unsigned)fcount(B.N, 1),
                  fc2 = (unsigned)fcount(B.N, 2);
   const float* Pf = B.Pf;
-  const float* pv[4] = {Pf + P_U * cells, Pf + P_V * cells, Pf + P_W * cells, Pf + P_T * cells};
+  // (u, v, w, T') of a cell: the first 16-byte group of its primitive record
+  auto phi = [&](unsigned cc, float* o) {
+    const float4 a = ld4(Pf + 4 * (size_t)cc);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+  };
   // float copies of the face vectors and volumes (the arithmetic is FP32; same values)
   const float* S0[3] = {B.Sf[0], B.Sf[0] + fc0, B.Sf[0] + 2 * (size_t)fc0};
   const float* S1[3] = {B.Sf[1], B.Sf[1] + fc1, B.Sf[1] + 2 * (size_t)fc1};
@@ -197,18 +196,14 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
   unsigned f1 = (unsigned)Ia * f1s0 + (unsigned)J * st1 + (unsigned)K;
   unsigned f2 = (unsigned)Ia * f2s0 + (unsigned)J * f2s1 + (unsigned)K;
   float phm[4], ph0[4], S0lo[3];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    phm[v] = ldu(pv[v], c - st0);
-    ph0[v] = ldu(pv[v], c);
-  }
+  phi(c - st0, phm);
+  phi(c, ph0);
 #pragma unroll
   for (int x = 0; x < 3; ++x) S0lo[x] = ldu(S0[x], f0);
 #pragma unroll 1
   for (int I = Ia; I < Ib; ++I) {
     float php[4], S0hi[3];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) php[v] = ldu(pv[v], c + st0);
+    phi(c + st0, php);
 #pragma unroll
     for (int x = 0; x < 3; ++x) S0hi[x] = ldu(S0[x], f0 + f0s0);
     float acc[12];
@@ -226,10 +221,13 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
         Slo[x] = ldu(S1[x], f1);
         Shi[x] = ldu(S1[x], f1 + st1);
       }
+      float pm[4], pp[4];
+      phi(c - st1, pm);
+      phi(c + st1, pp);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const float flo_v = h * (ldu(pv[v], c - st1) + ph0[v]);
-        const float fhi_v = h * (ph0[v] + ldu(pv[v], c + st1));
+        const float flo_v = h * (pm[v] + ph0[v]);
+        const float fhi_v = h * (ph0[v] + pp[v]);
 #pragma unroll
         for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
       }
@@ -241,10 +239,13 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
         Slo[x] = ldu(S2[x], f2);
         Shi[x] = ldu(S2[x], f2 + 1);
       }
+      float pm[4], pp[4];
+      phi(c - 1, pm);
+      phi(c + 1, pp);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const float flo_v = h * (ldu(pv[v], c - 1) + ph0[v]);
-        const float fhi_v = h * (ph0[v] + ldu(pv[v], c + 1));
+        const float flo_v = h * (pm[v] + ph0[v]);
+        const float fhi_v = h * (ph0[v] + pp[v]);
 #pragma unroll
         for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
       }
@@ -253,8 +254,8 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
     float g[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) g[i] = fdiv<float>(acc[i], volr);
-    const float rho = ldu((const float*)Pf, c);
-    const float mu = ldu(Pf + P_MU * cells, c);
+    const float4 pb = ld4(Pf + 4 * ((size_t)cells + c));     // (rho, p', mu, e')
+    const float rho = pb.x, mu = pb.z;
     float mut;
     if constexpr (SGS != WMX_SGS_NONE) {
       // filter width vol^(1/3) (solver.py:82), single-precision cube root
@@ -268,11 +269,13 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
     } else {
       mut = 0.0f * rho;
     }
-#pragma unroll
-    for (int i = 0; i < 12; ++i) stu(Gf + i * cells, c, g[i]);
-    stu(Gf + G_MU * cells, c, mu + mut);
-    stu(Gf + G_K * cells, c, (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t)));
-    stu(Gf + G_TH * cells, c, ducros<float>(g));
+    const size_t gq = 4 * (size_t)cells;                  // one float4 plane
+    float* grec = Gf + 4 * (size_t)c;
+    st4(grec, make_float4(g[0], g[1], g[2], g[3]));
+    st4(grec + gq, make_float4(g[4], g[5], g[6], g[7]));
+    st4(grec + 2 * gq, make_float4(g[8], g[9], g[10], g[11]));
+    st4(grec + 3 * gq, make_float4(mu + mut, (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t)),
+                                   ducros<float>(g), 0.0f));
 #pragma unroll
     for (int v = 0; v < 4; ++v) { phm[v] = ph0[v]; ph0[v] = php[v]; }
 #pragma unroll
