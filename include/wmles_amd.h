@@ -172,10 +172,11 @@ int wmx_plan_set_gauge(wmx_plan* plan, double p_ref, double T_ref);
 
 /* Kernel-class event timing (measurement aid, no reference counterpart).  Classes:
    0 flux+RK, 1 gradient/SGS, 2 primitives, 3 ghost fill, 4 wall model, 5 CFL dt,
-   6 mass flux, 7 ghost shell.  wmx_plan_profile(plan, mask) brackets every later
+   6 mass flux, 7 ghost shell, 8 inflow fill, 9 sponge mean.  wmx_plan_profile(plan, mask)
+   brackets every later
    launch of a class whose bit is set in `mask` with hipEvents on the launch stream
    (mask 0 stops and clears); wmx_plan_profile_read synchronises the recorded events
-   and returns per-class summed milliseconds and launch counts (8 entries each). */
+   and returns per-class summed milliseconds and launch counts (10 entries each). */
 int wmx_plan_profile(wmx_plan* plan, int32_t mask);
 int wmx_plan_profile_read(wmx_plan* plan, double* ms, int32_t* counts);
 

@@ -123,7 +123,7 @@ namespace {
 
 int nwg_of(const std::vector<int>& pre) { return pre.empty() ? 0 : pre.back(); }
 
-enum { K_FLUX = 0, K_GRAD, K_PRIM, K_FILL, K_WALL, K_DT, K_MF, K_SHELL, K_NCLASS };
+enum { K_FLUX = 0, K_GRAD, K_PRIM, K_FILL, K_WALL, K_DT, K_MF, K_SHELL, K_INFLOW, K_SPONGE, K_NCLASS };
 
 hipEvent_t next_event(wmx_plan* p) {
   if (p->ev_used == p->ev_pool.size()) {
@@ -408,8 +408,10 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
     for (int blk : p->phase_inflow[ph]) {
       const BlockDev& B = p->hb[blk];
       const int id = B.faces[0].inflow_id;
-      WMX_CHECK(launch_inflow(p->d_blks, blk, B.N[1], B.N[2], B.ng, p->inflows[id], q, p->nd.gas,
-                              time, d_dt, coef, s));
+      WMX_CHECK(timed(p, K_INFLOW, s, [&] {
+        return launch_inflow(p->d_blks, blk, B.N[1], B.N[2], B.ng, p->inflows[id], q, p->nd.gas, time,
+                             d_dt, coef, s);
+      }));
     }
   }
   return 0;
@@ -423,7 +425,9 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
   WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
   if (p->nd.n_sponge > 0)
     for (int b = 0; b < p->nb; ++b)
-      WMX_CHECK(launch_sponge_mean(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], q, s));
+      WMX_CHECK(timed(p, K_SPONGE, s, [&] {
+        return launch_sponge_mean(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], q, s);
+      }));
   return 0;
 }
 
@@ -618,7 +622,8 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     p->off_shell.push_back(off);
     off = align_up(off + sizeof(double) * 5 * B.shell_cells);
     p->off_sponge.push_back(off);
-    if (nd.n_sponge > 0) off = align_up(off + sizeof(double) * 5 * B.n[0] * B.n[1]);
+    if (nd.n_sponge > 0)
+      off = align_up(off + (sizeof(double) * 5 + sizeof(int)) * (size_t)B.n[0] * B.n[1]);
     for (int f = 0; f < 6; ++f)
       if (B.faces[f].kind == WMX_BC_WALLMODEL) {
         int t1, t2;
@@ -707,6 +712,9 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
     }
     B.shell = reinterpret_cast<double*>(p->ws + p->off_shell[b]);
     B.sponge_mean = reinterpret_cast<double*>(p->ws + p->off_sponge[b]);
+    B.sponge_row = p->nd.n_sponge > 0
+                       ? reinterpret_cast<int*>(B.sponge_mean + 5 * (size_t)B.n[0] * B.n[1])
+                       : nullptr;
     for (int f = 0; f < 6; ++f)
       B.wall[f] = p->off_wall[b][f] >= 0 ? reinterpret_cast<double*>(p->ws + p->off_wall[b][f]) : nullptr;
   }
@@ -714,6 +722,13 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
   p->d_delta = reinterpret_cast<double*>(p->ws + p->off_scal);
   p->d_stat = reinterpret_cast<double*>(p->ws + p->off_stat);
   WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
+  if (p->nd.n_sponge > 0) {
+    // rows of each block that need the sponge mean (static geometry; made once)
+    WMX_CHECK(hipDeviceSynchronize());
+    for (int b = 0; b < p->nb; ++b)
+      WMX_CHECK(launch_sponge_rows(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], p->nd, nullptr));
+    WMX_CHECK(hipDeviceSynchronize());
+  }
   if (p->nd.prec == WMX_PREC_FP32_BULK) {
     // float geometry copies, made once: the geometry may still be in flight on the
     // caller's stream, so the device is synchronised on both sides of the copies

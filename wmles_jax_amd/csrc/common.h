@@ -44,6 +44,7 @@ struct BlockDev {
   float* volf;              // FP32-bulk form: float copy of vol
   double* wall[6];          // WALL_PLANES planes of (n_t1, n_t2) per WallModel face
   double* sponge_mean;      // (5, n0, n1)
+  int* sponge_row;          // (n0, n1): 1 if some cell of the row has a sponge weight > 0
   double* shell;            // (5, shell cells) saved ghost values
   long long shell_cells;
   wmx_face_desc faces[6];
@@ -154,6 +155,8 @@ hipError_t launch_mfma_probe(const float* A, const float* B, float* D, hipStream
 int mlp_nparam();
 int mlp_nbf16();
 int mlp_nf32();
+hipError_t launch_sponge_rows(const BlockDev* d_blks, int blk, long long rows, const NumDev& nd,
+                              hipStream_t s);
 hipError_t launch_sponge_mean(const BlockDev* d_blks, int blk, long long rows, Ptrs q,
                               hipStream_t s);
 
@@ -207,6 +210,15 @@ __device__ __forceinline__ float2 ld2(const float* rec) {
 __device__ __forceinline__ void st4(float* rec, float4 v) {
   wmx_f4 w = {v.x, v.y, v.z, v.w};
   *(__attribute__((address_space(1))) wmx_f4*)__builtin_assume_aligned(rec, 16) = w;
+}
+
+// SmoothRampSponge weight at a cell centre x (smooth_ramp.py:212-221); pr = (x_start_l,
+// x_end_l, strength, U_inf, L)
+__host__ __device__ inline double sponge_sigma(const double* pr, double xcv) {
+  const double x_l = xcv / pr[4];
+  const double eta = fmin(fmax((x_l - pr[0]) / (pr[1] - pr[0]), 0.0), 1.0);
+  const double ramp = (eta * eta * eta) * (10.0 + eta * (-15.0 + 6.0 * eta));
+  return ((pr[2] * pr[3]) / pr[4]) * ramp;
 }
 
 // tangential axes of a face normal to `axis`, increasing order

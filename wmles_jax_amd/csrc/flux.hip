@@ -532,14 +532,13 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         const long long mrow = (long long)i * n1 + j;
         const long long mplane = (long long)n0 * n1;
         for (int sp = 0; sp < nd.n_sponge; ++sp) {
-          const double* pr = nd.sponge[sp];
-          const double x_l = xcv / pr[4];
-          const double eta = fmin(fmax((x_l - pr[0]) / (pr[1] - pr[0]), 0.0), 1.0);
-          const double ramp = (eta * eta * eta) * (10.0 + eta * (-15.0 + 6.0 * eta));
-          const double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
+          // a zero weight adds -0 * (q - mean): skipped (rows outside every sponge have no mean)
+          const double sigma = sponge_sigma(nd.sponge[sp], xcv);
+          if (sigma != 0.0) {
 #pragma unroll
-          for (int v = 0; v < 5; ++v)
-            r[v] = r[v] + (-sigma) * (vq[v] - ldu(B.sponge_mean + v * mplane, (unsigned)mrow));
+            for (int v = 0; v < 5; ++v)
+              r[v] = r[v] + (-sigma) * (vq[v] - ldu(B.sponge_mean + v * mplane, (unsigned)mrow));
+          }
         }
       }
       double* o = out.p[t.x];
@@ -743,14 +742,13 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
         const long long mrow = (long long)i * B.n[1] + j;
         const long long mplane = (long long)B.n[0] * B.n[1];
         for (int sp = 0; sp < nd.n_sponge; ++sp) {
-          const double* pr = nd.sponge[sp];
-          double x_l = xcv / pr[4];
-          double eta = fmin(fmax((x_l - pr[0]) / (pr[1] - pr[0]), 0.0), 1.0);
-          double ramp = (eta * eta * eta) * (10.0 + eta * (-15.0 + 6.0 * eta));
-          double sigma = ((pr[2] * pr[3]) / pr[4]) * ramp;
+          // a zero weight adds -0 * (q - mean): skipped (rows outside every sponge have no mean)
+          const double sigma = sponge_sigma(nd.sponge[sp], xcv);
+          if (sigma != 0.0) {
 #pragma unroll
-          for (int v = 0; v < 5; ++v)
-            r[v] = r[v] + (-sigma) * (vq[v] - gmem(B.sponge_mean)[v * mplane + mrow]);
+            for (int v = 0; v < 5; ++v)
+              r[v] = r[v] + (-sigma) * (vq[v] - gmem(B.sponge_mean)[v * mplane + mrow]);
+          }
         }
       }
       const gmut<double> o = gmem_w(out.p[t.x]);

@@ -247,21 +247,76 @@ __global__ void k_plane_final(int n1, int nch, double inv_count, const double* _
   }
 }
 
-// Volume-weighted spanwise (k) mean of the interior state per (i, j) row
-// (smooth_ramp.py:214-216), reduced in numpy's add.reduce order.  One thread per row.
-__global__ void k_sponge_mean(const BlockDev* __restrict__ blks, int blk, Ptrs q) {
+// Rows that need the sponge mean: some cell of the (i, j) row has a weight > 0 under
+// some sponge (the weight depends on the static cell centre only; computed once).
+__global__ void k_sponge_rows(const BlockDev* __restrict__ blks, int blk, NumDev nd) {
   const BlockDev& B = blks[blk];
-  long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long rows = (long long)B.n[0] * B.n[1];
   if (row >= rows) return;
   const int i = (int)(row / B.n[1]), j = (int)(row % B.n[1]);
-  const int ng = B.ng, nk = B.n[2];
+  const int ng = B.ng;
   const long long c0 = cidx(B.N, ng + i, ng + j, ng);
+  int need = 0;
+  for (int k = 0; k < B.n[2] && !need; ++k)
+    for (int sp = 0; sp < nd.n_sponge; ++sp)
+      if (sponge_sigma(nd.sponge[sp], B.xc[c0 + k]) != 0.0) need = 1;
+  B.sponge_row[row] = need;
+}
+
+// Volume-weighted spanwise (k) mean of the interior state per (i, j) row
+// (smooth_ramp.py:214-216) for the rows that need it, in numpy's add.reduce order: the
+// first element plus the pairwise sum of the rest, whose runs of <= 128 use 8 interleaved
+// accumulators.  Eight lanes take one row: lane u holds accumulator u, the three
+// combining levels are lane exchanges in the same association, lane 0 adds the tail.
+__global__ __launch_bounds__(256) void k_sponge_mean(const BlockDev* __restrict__ blks, int blk, Ptrs q) {
+  const BlockDev& B = blks[blk];
+  const long long rows = (long long)B.n[0] * B.n[1];
+  const long long row = (long long)blockIdx.x * 32 + (threadIdx.x >> 3);
+  const int u = threadIdx.x & 7;
+  const bool live = row < rows && B.sponge_row[row];
+  const int ng = B.ng, nk = B.n[2];
+  const long long c0 = live ? cidx(B.N, ng + (int)(row / B.n[1]), ng + (int)(row % B.n[1]), ng) : 0;
   const double* vol = B.vol + c0;
-  const double den = np_sum([&](int k) { return vol[k]; }, nk);
   const double* qb = q.p[blk];
+  const long long cells = B.cells;
+  const int n = nk - 1;                                  // the pairwise part: elements 1..nk-1
+  if (n <= 128 && n >= 8) {
+    double a[6];
+    const int nfull = n - n % 8;
+    if (live) {
+      const double vv = vol[1 + u];
+      a[0] = vv;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) a[1 + v] = qb[v * cells + c0 + 1 + u] * vv;
+      for (int i = 8; i < nfull; i += 8) {
+        const double w = vol[1 + i + u];
+        a[0] = a[0] + w;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i + u] * w;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 6; ++v) a[v] = 0.0;
+    }
+#pragma unroll
+    for (int m = 1; m <= 4; m <<= 1)
+#pragma unroll
+      for (int v = 0; v < 6; ++v) a[v] = a[v] + __shfl_xor(a[v], m, 8);
+    if (!live || u != 0) return;
+    for (int i = nfull; i < n; ++i) {
+      const double w = vol[1 + i];
+      a[0] = a[0] + w;
+      for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i] * w;
+    }
+    const double den = vol[0] + a[0];
+    for (int v = 0; v < 5; ++v) B.sponge_mean[v * rows + row] = (qb[v * cells + c0] * vol[0] + a[1 + v]) / den;
+    return;
+  }
+  if (!live || u != 0) return;                           // short or long rows: one lane per row
+  const double den = np_sum([&](int k) { return vol[k]; }, nk);
   for (int v = 0; v < 5; ++v) {
-    const double* qv = qb + v * B.cells + c0;
+    const double* qv = qb + v * cells + c0;
     const double num = np_sum([&](int k) { return qv[k] * vol[k]; }, nk);
     B.sponge_mean[v * rows + row] = num / den;
   }
@@ -319,8 +374,14 @@ hipError_t launch_mf_apply(const BlockDev* d_blks, const int* d_pre, int nblk, i
 hipError_t launch_sponge_mean(const BlockDev* d_blks, int blk, long long rows, Ptrs q,
                               hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sponge_mean, dim3((unsigned)((rows + 127) / 128)), dim3(128), 0, s, d_blks,
-                     blk, q);
+  hipLaunchKernelGGL(k_sponge_mean, dim3((unsigned)((rows + 31) / 32)), dim3(256), 0, s, d_blks, blk, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_sponge_rows(const BlockDev* d_blks, int blk, long long rows, const NumDev& nd,
+                              hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sponge_rows, dim3((unsigned)((rows + 127) / 128)), dim3(128), 0, s, d_blks, blk, nd);
   return hipGetLastError();
 }
 

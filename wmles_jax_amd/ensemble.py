@@ -167,7 +167,7 @@ class EnsembleSolver:
             q.record_stream(self._stream)
         return self._split(out), dts
 
-    KERNEL_CLASSES = ("flux", "grad", "prim", "fill", "wall", "dt", "massflux", "shell")
+    KERNEL_CLASSES = ("flux", "grad", "prim", "fill", "wall", "dt", "massflux", "shell", "inflow", "sponge")
 
     def profile(self, classes=("flux",)):
         """Start HIP-event timing of the named kernel classes (empty: stop).  Launches
@@ -179,8 +179,8 @@ class EnsembleSolver:
 
     def profile_read(self):
         """{class: (total_ms, launches)} of the launches timed since profile()."""
-        ms = (C.c_double * 8)()
-        cnt = (C.c_int32 * 8)()
+        ms = (C.c_double * len(self.KERNEL_CLASSES))()
+        cnt = (C.c_int32 * len(self.KERNEL_CLASSES))()
         _lib.check(_lib.lib.wmx_plan_profile_read(self._plan, ms, cnt))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES) if cnt[i]}
 

@@ -228,7 +228,7 @@ class Solver:
             raise ValueError("one state array per block is required")
         return tuple(out)
 
-    KERNEL_CLASSES = ("flux", "grad", "prim", "fill", "wall", "dt", "massflux", "shell")
+    KERNEL_CLASSES = ("flux", "grad", "prim", "fill", "wall", "dt", "massflux", "shell", "inflow", "sponge")
 
     def profile(self, classes=("flux",)):
         """Start event timing of the named kernel classes (empty: stop)."""
@@ -239,8 +239,8 @@ class Solver:
 
     def profile_read(self):
         """{class: (total_ms, launches)} of the launches timed since profile()."""
-        ms = (C.c_double * 8)()
-        cnt = (C.c_int32 * 8)()
+        ms = (C.c_double * len(self.KERNEL_CLASSES))()
+        cnt = (C.c_int32 * len(self.KERNEL_CLASSES))()
         _lib.check(_lib.lib.wmx_plan_profile_read(self._plan, ms, cnt))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_CLASSES) if cnt[i]}
 
