@@ -563,7 +563,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     }
   for (int i = 0; i < ninflow; ++i) {
     const wmx_inflow_desc& in = inflows[i];
-    p->inflows.push_back(InflowDev{in.n_tab, in.n_modes, in.table, in.modes, in.U_inf, in.L,
+    p->inflows.push_back(InflowDev{in.n_tab, in.n_modes, 0, in.table, in.modes, in.U_inf, in.L,
                                    in.rho_inf, in.p_inf});
   }
   p->passes = effective_passes(*p, nm->bc_passes);

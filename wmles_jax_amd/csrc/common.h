@@ -94,6 +94,7 @@ struct WallOp {
 
 struct InflowDev {
   int n_tab, n_modes;
+  int coop;                 // set by the launcher: phases evaluated cooperatively in LDS
   const double* table;
   const double* modes;
   double U_inf, L, rho_inf, p_inf;

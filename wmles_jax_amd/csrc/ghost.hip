@@ -239,21 +239,49 @@ __global__ void k_inflow(const BlockDev* __restrict__ blks, int blk, InflowDev i
   double z_l = B.xc[2 * cells + ci] / in.L;
   double X0 = in.U_inf * t / in.L;
   const bool inner = valid && K >= ng && K < ng + nk;
+  const int nm = in.n_modes;
+  // cos of every (cell, mode) phase, computed by all threads of the workgroup into LDS
+  // (when the row fits), then summed per cell in mode order: the same values and the same
+  // sum as one thread per cell doing its modes in turn
+  double* cosb = red + blockDim.x + 1;                  // [N2][nm] when cooperative
+  double* yz = cosb + (long long)N2 * nm;               // [2][N2]
+  const bool coop = in.coop;
+  if (coop) {
+    for (int kk = threadIdx.x; kk < N2; kk += blockDim.x) {
+      const long long cj = cidx(B.N, gi, J, kk);
+      yz[kk] = B.xc[cells + cj] / in.L;
+      yz[N2 + kk] = B.xc[2 * cells + cj] / in.L;
+    }
+  }
   double fld[3];
   for (int comp = 0; comp < 3; ++comp) {
-    const double* md = in.modes + (long long)comp * 4 * in.n_modes;
+    const double* md = in.modes + (long long)comp * 4 * nm;
     double f = 0.0;
-    if (valid) {
+    if (coop) {
+      __syncthreads();                                   // yz staged / previous sums done
+      for (int e = threadIdx.x; e < N2 * nm; e += blockDim.x) {
+        const int kk = e / nm, k = e - kk * nm;
+        const double ph = ((md[k] * X0 + md[nm + k] * yz[kk]) + md[2 * nm + k] * yz[N2 + kk]) + md[3 * nm + k];
+        cosb[e] = cos(ph);
+      }
+      __syncthreads();
+      if (valid) {
+        const double* cr = cosb + (long long)K * nm;
+        double acc = cr[0];
+        for (int k = 1; k < nm; ++k) acc = acc + cr[k];
+        f = acc;
+      }
+    } else if (valid) {
       double acc = 0.0;
-      for (int k = 0; k < in.n_modes; ++k) {
-        double ph = ((md[k] * X0 + md[in.n_modes + k] * y_l) + md[2 * in.n_modes + k] * z_l) +
-                    md[3 * in.n_modes + k];
+      for (int k = 0; k < nm; ++k) {
+        double ph = ((md[k] * X0 + md[nm + k] * y_l) + md[2 * nm + k] * z_l) + md[3 * nm + k];
         acc = (k == 0) ? cos(ph) : acc + cos(ph);
       }
       f = acc;
     }
     fld[comp] = f;
   }
+  if (coop) __syncthreads();                             // the row statistics reuse red[]
   // row statistics over the interior span, summed in numpy's add.reduce order
   auto row_mean = [&](double val) {
     red[K] = inner ? val : 0.0;
@@ -320,7 +348,11 @@ hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng
   int threads = ((N2 + 63) / 64) * 64;
   if (threads > 1024) return hipErrorInvalidValue;
   dim3 grid(N1, ng);
-  size_t shm = sizeof(double) * (threads + 1);
+  // cooperative phase evaluation when a row's (cell, mode) table fits in 56 KB of LDS
+  const size_t coop_bytes = sizeof(double) * ((size_t)N2 * in.n_modes + 2 * (size_t)N2);
+  in.coop = coop_bytes <= 56 * 1024 ? 1 : 0;
+  if (in.coop && threads < 256) threads = 256;           // more lanes for the phase table
+  size_t shm = sizeof(double) * (threads + 1) + (in.coop ? coop_bytes : 0);
   hipLaunchKernelGGL(k_inflow, grid, dim3(threads), shm, s, d_blks, blk, in, q, gas, time, d_dt,
                      coef);
   return hipGetLastError();
