@@ -223,6 +223,16 @@ constexpr int C2J = FLUX2_TILE_J;              // cell rows of an FP32-bulk tile
 constexpr int H2J = C2J + 4, H2K = TK + 4;     // primitive region (halo 2)
 constexpr int G2J = C2J + 2, G2K = TK + 2;     // gradient / centroid region (halo 1)
 constexpr int NP2 = H2J * H2K, NG2 = G2J * G2K;
+// LDS banks: a wavefront covers tile rows w, w+4, w+8, w+12 (w = wavefront index) with
+// 16 columns each.  Rows of pitch 20 then start 80 words apart, 16 banks apart modulo 32
+// (and 16, 32, 48 modulo 64), so every row-of-16 access of a wavefront is conflict-free.
+// The gradient / centroid region is stored with that pitch too (18 columns used).
+constexpr int G2P = 20, NG2S = G2J * G2P;
+static_assert(H2K == 20 && G2P >= G2K && TK == 16 && NT == 256, "bank-free layout assumes 16 x 16");
+// face exchange slots: a face with tile coordinates (r, c), c < 16, sits at the index of
+// the thread that owns (r, c); the upper edge k faces (c = 16) follow at 256 + r
+__device__ __forceinline__ int tile_tid(int r, int c) { return (r & 3) * 64 + (r >> 2) * 16 + c; }
+__device__ __forceinline__ int kface_slot(int r, int c) { return c < TK ? tile_tid(r, c) : NT + r; }
 
 // slot t of the halo ring of width h around a rows x TK interior
 __device__ __forceinline__ void ring_slot(int t, int h, int rows, int* sj, int* sk) {
@@ -243,7 +253,7 @@ __device__ __forceinline__ CellWT<R> sp_w(const R (*sP)[NP2], int s) {
 }
 
 template <class R>
-__device__ __forceinline__ CellGT<R> sg_g(const R (*sG)[NG2], int s) {
+__device__ __forceinline__ CellGT<R> sg_g(const R (*sG)[NG2S], int s) {
   CellGT<R> g;
 #pragma unroll
   for (int i = 0; i < 12; ++i) g.g[i] = sG[i][s];
@@ -263,18 +273,18 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   // staging area: primitives (halo 2) and gradient records (halo 1) in R; the FP64
   // face fluxes of the plane go to their own exchange buffers as they are computed
   __shared__ R sP[NPW][NP2];
-  __shared__ R sG[NGW][NG2];
-  __shared__ double sX[3][NG2];
+  __shared__ R sG[NGW][NG2S];
+  __shared__ double sX[3][NG2S];
   // face exchange: F'[5] and w (gauge form, FP32)
-  __shared__ float sFj[NFX][(C2J + 1) * TK];
-  __shared__ float sFk[NFX][C2J * (TK + 1)];
-  __shared__ float sFi[2][NFX][C2J * TK];        // own i faces: lower / upper, ping-pong
+  __shared__ float sFj[NFX][NT];                 // slot tile_tid(r, c)
+  __shared__ float sFk[NFX][NT + C2J + 1];       // slot kface_slot(r, c)
+  __shared__ float sFi[2][NFX][C2J * TK];        // own i faces by thread: lower / upper, ping-pong
   const int4 t = tiles[blockIdx.x];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
   const int i1 = min(i0 + ci_len, B.n[0]);
   const int tid = threadIdx.x;
-  const int tj = tid / TK, tk = tid % TK;
+  const int tj = (tid >> 6) + 4 * ((tid >> 4) & 3), tk = tid & 15;   // tile_tid(tj, tk) == tid
   const int ng = B.ng;
   const int* N = B.N;
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
@@ -310,7 +320,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
   Gw[1] = load_colg32(G, cells, (size_t)(I0 - 1) * sI + cb);
 #pragma unroll
   for (int x = 0; x < 3; ++x) Xw[1][x] = ldu(XC + x * cells, (unsigned)(I0 - 1) * sI + cb);
-  const int own_s = tj * TK + tk;               // this column's i-face slot (own threads)
+  const int own_s = tid;                        // this column's i-face slot (own threads: tid < 240)
   int lo = 0;                                    // slot of the current lower i face
   // halo ring job of this thread (plane-independent): 140 primitive slots (halo 2),
   // 66 gradient/centroid slots (halo 1); prefetched one plane ahead into registers
@@ -326,7 +336,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     int sj, sk;
     ring_slot(tid - NRP, 1, C2J, &sj, &sk);
     const int jj = ng + j0 - 1 + sj, kk = ng + k0 - 1 + sk;
-    if (g_ok(jj, kk)) { ring = 2; ring_s = sj * G2K + sk; ring_off = (unsigned)jj * N[2] + kk; }
+    if (g_ok(jj, kk)) { ring = 2; ring_s = sj * G2P + sk; ring_off = (unsigned)jj * N[2] + kk; }
   }
   R rv[NGW];
   double rx[3];
@@ -355,7 +365,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
     // ---- stage plane I in LDS: own columns from registers, halo and idle columns from
     //      HBM (each group of loads is issued as a batch before its LDS stores)
     if (!first) {
-      const int sp = (tj + 2) * H2K + (tk + 2), sg = (tj + 1) * G2K + (tk + 1);
+      const int sp = (tj + 2) * H2K + (tk + 2), sg = (tj + 1) * G2P + (tk + 1);
       const unsigned pl = (unsigned)I * sI;
       if (act) {
         sP[0][sp] = Pw[1].w.r; sP[1][sp] = Pw[1].w.u; sP[2][sp] = Pw[1].w.v; sP[3][sp] = Pw[1].w.w;
@@ -430,7 +440,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
 #pragma unroll
           for (int x = 0; x < 3; ++x) xL[x] = Xw[0][x];
         } else {                                     // plane I of the own column, staged
-          const int sg = (tj + 1) * G2K + (tk + 1);
+          const int sg = (tj + 1) * G2P + (tk + 1);
           gL = sg_g<R>(sG, sg); thL = sG[14][sg];
 #pragma unroll
           for (int x = 0; x < 3; ++x) xL[x] = sX[x][sg];
@@ -450,8 +460,8 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         fJ = ng + j0 + rj;
         fK = ng + k0 + rk;
         const int dj = d == 1, dk = d == 2;
-        const int pR = (rj + 2) * H2K + (rk + 2), gR_ = (rj + 1) * G2K + (rk + 1);
-        const int pst = dj * H2K + dk, gst = dj * G2K + dk;
+        const int pR = (rj + 2) * H2K + (rk + 2), gR_ = (rj + 1) * G2P + (rk + 1);
+        const int pst = dj * H2K + dk, gst = dj * G2P + dk;
         if (doit) {
           L2 = sp_w<R>(sP, pR - 2 * pst); L = sp_w<R>(sP, pR - pst); Rc = sp_w<R>(sP, pR);
           R2 = sp_w<R>(sP, pR + pst);
@@ -489,10 +499,10 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           for (int v = 0; v < NFX; ++v) sFi[hs][v][own_s] = F[v];
         } else if (d == 1) {
 #pragma unroll
-          for (int v = 0; v < NFX; ++v) sFj[v][rj * TK + rk] = F[v];
+          for (int v = 0; v < NFX; ++v) sFj[v][tile_tid(rj, rk)] = F[v];
         } else {
 #pragma unroll
-          for (int v = 0; v < NFX; ++v) sFk[v][rj * (TK + 1) + rk] = F[v];
+          for (int v = 0; v < NFX; ++v) sFk[v][kface_slot(rj, rk)] = F[v];
         }
       }
     }
@@ -515,11 +525,12 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
       // outward sums in FP64 of the FP32 face values; the energy gets the gauge terms
       // e_ref x (mass divergence) + p_ref x (sum of w)
       double a[NFX];
+      const int jup = tile_tid(tj + 1, tk), kup = kface_slot(tj, tk + 1);
 #pragma unroll
       for (int v = 0; v < NFX; ++v) {
         double t = 0.0 - ((double)sFi[lo ^ 1][v][own_s] - (double)sFi[lo][v][own_s]);
-        t = t - ((double)sFj[v][(tj + 1) * TK + tk] - (double)sFj[v][tj * TK + tk]);
-        t = t - ((double)sFk[v][tj * (TK + 1) + tk + 1] - (double)sFk[v][tj * (TK + 1) + tk]);
+        t = t - ((double)sFj[v][jup] - (double)sFj[v][tid]);
+        t = t - ((double)sFk[v][kup] - (double)sFk[v][tid]);
         a[v] = t;
       }
       a[4] = (a[4] + nd.e_ref * a[0]) + nd.p_ref * a[5];
