@@ -149,7 +149,7 @@ def test_tile_model_picks_the_shortest_makespan():
     def brute(dims, prec, cus=256):
         tj, alpha = 15, (1.5 if prec == _lib.PREC_FP32_BULK else 0.5)
         best = None
-        for ci in range(32, 3, -1):
+        for ci in range(64, 3, -1):
             T = sum(-(-dims[3 * b] // ci) * -(-dims[3 * b + 1] // tj) * -(-dims[3 * b + 2] // 16)
                     for b in range(len(dims) // 3))
             cost = -(-T // (2 * cus)) * (ci + alpha)
@@ -161,6 +161,9 @@ def test_tile_model_picks_the_shortest_makespan():
                  [100, 30, 17]):
         for prec in (_lib.PREC_FP64, _lib.PREC_FP32_BULK):
             assert model(dims, prec) == brute(dims, prec)
-    # the 256 x 128 x 128 channel: 72 tiles of 15 x 16 per i chunk; 14 chunks of 19 planes
-    # make 1008 tiles, two rounds of 512 slots
-    assert model([256, 128, 128], _lib.PREC_FP32_BULK) == (19, 1008)
+    # the 256 x 128 x 128 channel: 72 tiles of 15 x 16 per i chunk; 7 chunks of 37 planes
+    # make 504 tiles, one round of 512 slots (19 planes: 1008 tiles, two rounds, 41 plane
+    # times against 38.5)
+    assert model([256, 128, 128], _lib.PREC_FP32_BULK) == (37, 504)
+    # the 512 x 256 x 256 channel keeps 32 planes: 4608 tiles, nine rounds
+    assert model([512, 256, 256], _lib.PREC_FP32_BULK) == (32, 4608)

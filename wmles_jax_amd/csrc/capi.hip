@@ -317,7 +317,7 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // workgroups per CU, so a launch of T tiles takes ceil(T / (2 CUs)) rounds of
 // (ci + alpha) plane times, alpha being the per-tile start-up (the FP32-bulk kernel's
 // register window costs about 1.5 planes, the FP64 kernel's about 0.5).  The shortest
-// estimate over ci in [4, 32] wins (ties: the longer chunk).  dims: n[3] per block.
+// estimate over ci in [4, 64] wins (ties: the longer chunk).  dims: n[3] per block.
 int auto_ci_len(int nb, const int* dims, int prec, int cu_count, long long* ntiles) {
   const int tjr = FLUX2_TILE_J;
   const double alpha = prec == WMX_PREC_FP32_BULK ? 1.5 : 0.5;
@@ -325,7 +325,7 @@ int auto_ci_len(int nb, const int* dims, int prec, int cu_count, long long* ntil
   int best = 16;
   long long best_T = 0;
   double best_cost = 1e300;
-  for (int ci = 32; ci >= 4; --ci) {
+  for (int ci = 64; ci >= 4; --ci) {
     long long T = 0;
     for (int b = 0; b < nb; ++b) {
       const int* n = dims + 3 * b;
