@@ -100,6 +100,9 @@ struct wmx_plan {
   // case tables over the interior-cell workgroups
   std::vector<int> case_wg, case_off;
   int *d_case_wg = nullptr, *d_case_off = nullptr, *d_case_dir = nullptr;
+  // the same per-case workgroup lists for the all-cells layout (fused stage-1 prim + dt)
+  std::vector<int> case_wg_all, case_off_all;
+  int *d_case_wg_all = nullptr, *d_case_off_all = nullptr;
   double* d_case_target = nullptr;
   // inflows
   std::vector<InflowDev> inflows;
@@ -418,9 +421,20 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
 }
 
 // ghost fill + cell-level fields of one RHS evaluation
-int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s) {
+// dt_out != nullptr: the primitive launch also takes the CFL partials of the (unchanged)
+// interior state and the per-case minimum goes to dt_out before the gradients
+int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s,
+              double* dt_out = nullptr) {
   if (run_fill(p, q, time, d_dt, coef, s)) return 1;
-  WMX_CHECK(timed(p, K_PRIM, s, [&] { return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd, s); }));
+  WMX_CHECK(timed(p, K_PRIM, s, [&] {
+    return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd,
+                       dt_out ? p->d_part : nullptr, s);
+  }));
+  if (dt_out)
+    WMX_CHECK(timed(p, K_DT, s, [&] {
+      return launch_case_final(0, p->n_cases, p->d_case_wg_all, p->d_case_off_all, p->d_part, nullptr,
+                               dt_out, s);
+    }));
   WMX_CHECK(timed(p, K_GRAD, s, [&] { return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s); }));
   WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
   if (p->nd.n_sponge > 0)
@@ -605,6 +619,13 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
         for (int w = p->pre_int[b]; w < p->pre_int[b + 1]; ++w) p->case_wg.push_back(w);
     p->case_off.push_back((int)p->case_wg.size());
   }
+  p->case_off_all.assign(1, 0);
+  for (int c = 0; c < p->n_cases; ++c) {
+    for (int b = 0; b < nblocks; ++b)
+      if (p->hb[b].case_id == c)
+        for (int w = p->pre_all[b]; w < p->pre_all[b + 1]; ++w) p->case_wg_all.push_back(w);
+    p->case_off_all.push_back((int)p->case_wg_all.size());
+  }
   // workspace layout
   size_t off = 0;
   p->off_wall.assign(nblocks, std::vector<long long>(6, -1));
@@ -633,7 +654,9 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
       }
   }
   p->off_part = off;
-  off = align_up(off + sizeof(double) * 2 * (size_t)std::max(1, nwg_of(p->pre_int)));
+  // partials: two per interior workgroup (mass flux), one per all-cells workgroup (fused dt)
+  off = align_up(off + sizeof(double) * (size_t)std::max(std::max(1, 2 * nwg_of(p->pre_int)),
+                                                         nwg_of(p->pre_all)));
   p->off_scal = off;
   off = align_up(off + sizeof(double) * 64);
   {
@@ -649,6 +672,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
       upload(p->pre_all, &p->d_pre_all) || upload(p->pre_box, &p->d_pre_box) ||
       upload(p->pre_int, &p->d_pre_int) || upload(p->pre_shell, &p->d_pre_shell) ||
       upload(p->case_wg, &p->d_case_wg) || upload(p->case_off, &p->d_case_off) ||
+      upload(p->case_wg_all, &p->d_case_wg_all) || upload(p->case_off_all, &p->d_case_off_all) ||
       upload(cdir, &p->d_case_dir) || upload(p->case_target, &p->d_case_target)) {
     std::string m = g_err;
     wmx_plan_destroy(p);
@@ -676,6 +700,8 @@ int wmx_plan_destroy(wmx_plan* p) {
   (void)hipFree(p->d_pre_shell);
   (void)hipFree(p->d_case_wg);
   (void)hipFree(p->d_case_off);
+  (void)hipFree(p->d_case_wg_all);
+  (void)hipFree(p->d_case_off_all);
   (void)hipFree(p->d_case_dir);
   (void)hipFree(p->d_case_target);
   delete p;
@@ -906,9 +932,11 @@ int wmx_mass_flux(wmx_plan* p, double* const* q, void* stream) {
   return 0;
 }
 
-int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, const double* d_dt,
-                double time, void* stream) {
-  if (check_plan(p)) return 1;
+namespace {
+// fuse_dt: d_dt is computed here, from q0, by the stage-1 primitive launch (the fill before
+// it changes ghosts only, and reads d_dt only for inflow faces, which never fuse)
+int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt, double* d_dt,
+                 double time, void* stream, bool fuse_dt) {
   for (int b = 0; b < p->nb; ++b)
     if (q0[b] == qout[b]) return fail("qout must not alias q0");
   hipStream_t s = as_stream(stream);
@@ -917,7 +945,7 @@ int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, 
   WMX_CHECK(timed(p, K_SHELL, s, [&] { return launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s); }));
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
-  if (run_cells(p, P0, time, d_dt, 0.0, s)) return 1;
+  if (run_cells(p, P0, time, d_dt, 0.0, s, fuse_dt ? d_dt : nullptr)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
@@ -927,6 +955,13 @@ int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, 
     return launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s);
   }));
   return wmx_mass_flux(p, qout, stream);
+}
+}  // namespace
+
+int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, const double* d_dt,
+                double time, void* stream) {
+  if (check_plan(p)) return 1;
+  return advance_impl(p, q0, qout, dt, const_cast<double*>(d_dt), time, stream, false);
 }
 
 int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
@@ -982,10 +1017,14 @@ int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, 
   hipStream_t s = as_stream(stream);
   bool has_inflow = false;
   for (auto& ph : p->phase_inflow) has_inflow |= !ph.empty();
-  if (!use_graph || has_inflow || s == nullptr) {
+  for (int b = 0; b < p->nb; ++b)
+    if (q0[b] == qout[b]) return fail("qout must not alias q0");
+  // without inflow faces the step's dt comes from its stage-1 primitive launch
+  if (has_inflow) {
     if (wmx_max_dt(p, q0, d_dt, stream)) return 1;
     return wmx_advance(p, q0, qout, 0.0, d_dt, time, stream);
   }
+  if (!use_graph || s == nullptr) return advance_impl(p, q0, qout, 0.0, d_dt, time, stream, true);
   for (auto& g : p->graphs) {
     bool same = g.d_dt == d_dt && g.time == time;
     for (int b = 0; same && b < p->nb; ++b) same = g.q0[b] == q0[b] && g.qout[b] == qout[b];
@@ -997,8 +1036,7 @@ int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, 
   hipGraph_t graph;
   WMX_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   p->capturing = true;
-  int rc = wmx_max_dt(p, q0, d_dt, stream);
-  if (!rc) rc = wmx_advance(p, q0, qout, 0.0, d_dt, time, stream);
+  int rc = advance_impl(p, q0, qout, 0.0, d_dt, time, stream, true);
   p->capturing = false;
   hipError_t e = hipStreamEndCapture(s, &graph);
   if (rc) return 1;

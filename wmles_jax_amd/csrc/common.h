@@ -111,8 +111,9 @@ hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng
 hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);
+// d_part != nullptr: the launch also writes the CFL partial of each workgroup (step fusion)
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
-                       const NumDev& nd, hipStream_t s);
+                       const NumDev& nd, double* d_part, hipStream_t s);
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
                        hipStream_t s);
 hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, int maxcount, Ptrs q,

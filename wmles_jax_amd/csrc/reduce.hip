@@ -1,7 +1,10 @@
 // Whole-domain reductions of the step: CFL time step (solver.py:135-154), the
 // constant-mass-flux constraint (solver.py:16-36) and the sponge's spanwise mean
 // (smooth_ramp.py:214-216).  Two-level: per-workgroup partials (wavefront shuffles
-// + LDS), then one workgroup per ensemble case.  Built with -ffp-contract=off.
+// + LDS), then one workgroup per ensemble case.  Also the primitive kernels
+// (gas.py:24-33), whose stage-1 launch of a step can take the CFL partials, so the
+// separate dt kernel and the fused one evaluate the same code.  Built with
+// -ffp-contract=off.
 #include "common.h"
 
 namespace wmx {
@@ -48,6 +51,46 @@ __device__ __forceinline__ unsigned interior_cell(const BlockDev& B, unsigned l,
   return ((unsigned)*I * (unsigned)B.N[1] + (unsigned)*J) * (unsigned)B.N[2] + (unsigned)*K;
 }
 
+// CFL time step of interior cell (I, J, K) (ghosted index c) from its primitives w;
+// solver.py:139-150.  Every load first (one memory round trip), then the arithmetic.  The
+// dt kernel and the fused stage-1 primitive kernels below share it in this unit, so the
+// two give the same bits.
+__device__ __forceinline__ double cell_dt(const BlockDev& B, const NumDev& nd, unsigned c, int I,
+                                          int J, int K, const Prim& w) {
+  double Sv[3][3][2];
+  const double vol = ldu(B.vol, c);
+  const int cc[3] = {I, J, K};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double* S = B.S[d];
+    const unsigned fc = (unsigned)fcount(B.N, d);
+    int hi[3] = {cc[0], cc[1], cc[2]};
+    hi[d] += 1;
+    const unsigned f0 = (unsigned)fidx(B.N, d, cc[0], cc[1], cc[2]);
+    const unsigned f1 = (unsigned)fidx(B.N, d, hi[0], hi[1], hi[2]);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      Sv[d][x][0] = ldu(S + x * fc, f0);
+      Sv[d][x][1] = ldu(S + x * fc, f1);
+    }
+  }
+  const GasC& gas = nd.gas;
+  const double cs = sqrt(gas.gamma * gas.R * w.T);
+  const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
+  const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
+  double lam = 0.0, lamv = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double s0 = 0.5 * (Sv[d][0][0] + Sv[d][0][1]);
+    const double s1 = 0.5 * (Sv[d][1][0] + Sv[d][1][1]);
+    const double s2 = 0.5 * (Sv[d][2][0] + Sv[d][2][1]);
+    const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
+    lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
+    lamv = lamv + (nu * (Sm * Sm)) / vol;
+  }
+  return nd.cfl / ((lam + 4.0 * lamv) / vol);
+}
+
 // Per-cell CFL time step, min over interiors; solver.py:139-150.
 __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__ blks,
                                                     const int* __restrict__ wg_prefix, int nblk,
@@ -63,45 +106,97 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
     const unsigned c = interior_cell(B, l, &I, &J, &K);
     const unsigned cells = (unsigned)B.cells;
     const double* qb = q.p[b];
-    const GasC& gas = nd.gas;
-    // every load of the cell first (one memory round trip), then the arithmetic
-    double qv[5], Sv[3][3][2];
+    double qv[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) qv[v] = ldu(qb + v * cells, c);
-    const double vol = ldu(B.vol, c);
-    const int cc[3] = {I, J, K};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double* S = B.S[d];
-      const unsigned fc = (unsigned)fcount(B.N, d);
-      int hi[3] = {cc[0], cc[1], cc[2]};
-      hi[d] += 1;
-      const unsigned f0 = (unsigned)fidx(B.N, d, cc[0], cc[1], cc[2]);
-      const unsigned f1 = (unsigned)fidx(B.N, d, hi[0], hi[1], hi[2]);
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        Sv[d][x][0] = ldu(S + x * fc, f0);
-        Sv[d][x][1] = ldu(S + x * fc, f1);
-      }
-    }
-    Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
-    const double cs = sqrt(gas.gamma * gas.R * w.T);
-    const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
-    const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
-    double lam = 0.0, lamv = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double s0 = 0.5 * (Sv[d][0][0] + Sv[d][0][1]);
-      const double s1 = 0.5 * (Sv[d][1][0] + Sv[d][1][1]);
-      const double s2 = 0.5 * (Sv[d][2][0] + Sv[d][2][1]);
-      const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
-      lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
-      lamv = lamv + (nu * (Sm * Sm)) / vol;
-    }
-    dtc = nd.cfl / ((lam + 4.0 * lamv) / vol);
+    const Prim w = cons_to_prim(nd.gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
+    dtc = cell_dt(B, nd, c, I, J, K, w);
   }
   const double r = block_reduce<true>(dtc);
   if (threadIdx.x == 0) part[wg] = r;
+}
+
+// ------------------------------------------------------------------------------ prim
+// Primitive records of every cell (ghosts included).  DT: the stage-1 launch of a step
+// also takes the CFL partial of its workgroup's interior cells into part[workgroup],
+// reusing the state it loads (the per-case min then follows as for k_dt_partial).
+__device__ __forceinline__ bool interior_of(const BlockDev& B, unsigned c, int* I, int* J, int* K) {
+  const unsigned N1 = (unsigned)B.N[1], N2 = (unsigned)B.N[2];
+  const unsigned r = c / N2;
+  *K = (int)(c - r * N2);
+  *J = (int)(r % N1);
+  *I = (int)(r / N1);
+  const int ng = B.ng;
+  return *I >= ng && *I < ng + B.n[0] && *J >= ng && *J < ng + B.n[1] && *K >= ng &&
+         *K < ng + B.n[2];
+}
+
+// FP64 record (r, u, v, w, p, T, mu, e_int)
+template <bool DT>
+__global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
+                                              const int* __restrict__ wg_prefix, int nblk,
+                                              Ptrs q, NumDev nd, double* __restrict__ part) {
+  // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
+  // stay in one L2
+  const int wg = xcd_span(blockIdx.x, gridDim.x);
+  const int b = find_block_r(wg_prefix, nblk, wg);
+  const BlockDev& B = blks[b];
+  const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const long long cells = B.cells;
+  double dtc = __builtin_huge_val();
+  if (c < cells) {
+    const GasC& gas = nd.gas;
+    const double* qb = q.p[b];
+    const Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
+                                qb[4 * cells + c]);
+    double* P = B.P;
+    P[c] = w.r;
+    P[cells + c] = w.u;
+    P[2 * cells + c] = w.v;
+    P[3 * cells + c] = w.w;
+    P[4 * cells + c] = w.p;
+    P[5 * cells + c] = w.T;
+    P[6 * cells + c] = mu_suth(gas, w.T);
+    P[7 * cells + c] = w.p / ((gas.gamma - 1.0) * w.r);
+    int I, J, K;
+    if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
+  }
+  if (DT) {
+    const double r = block_reduce<true>(dtc);
+    if (threadIdx.x == 0) part[wg] = r;
+  }
+}
+
+// FP32-bulk record: prims evaluated in FP64, stored in FP32 with p, T and e_int
+// shifted by the block gauge so their small variations keep FP32 precision.
+template <bool DT>
+__global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blks,
+                                                const int* __restrict__ wg_prefix, int nblk,
+                                                Ptrs q, NumDev nd, double* __restrict__ part) {
+  // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
+  // stay in one L2
+  const int wg = xcd_span(blockIdx.x, gridDim.x);
+  const int b = find_block_r(wg_prefix, nblk, wg);
+  const BlockDev& B = blks[b];
+  const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const long long cells = B.cells;
+  double dtc = __builtin_huge_val();
+  if (c < cells) {
+    const GasC& gas = nd.gas;
+    const double* qb = q.p[b];
+    const Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
+                                qb[4 * cells + c]);
+    st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - nd.T_gauge)));
+    st4(B.Pf + 4 * ((size_t)cells + c),
+        make_float4((float)w.r, (float)(w.p - nd.p_ref), (float)mu_suth(gas, w.T),
+                    (float)(w.p / ((gas.gamma - 1.0) * w.r) - nd.e_ref)));
+    int I, J, K;
+    if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
+  }
+  if (DT) {
+    const double r = block_reduce<true>(dtc);
+    if (threadIdx.x == 0) part[wg] = r;
+  }
 }
 
 // sum over interiors of rho u_dir vol (and vol); solver.py:23-27
@@ -327,6 +422,20 @@ hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk,
                              const NumDev& nd, double* d_part, hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_dt_partial, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+  return hipGetLastError();
+}
+
+hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
+                       const NumDev& nd, double* d_part, hipStream_t s) {
+  if (nwg <= 0) return hipSuccess;
+  const bool f32 = nd.prec == WMX_PREC_FP32_BULK;
+  if (d_part) {
+    if (f32) hipLaunchKernelGGL(k_prim32<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+    else hipLaunchKernelGGL(k_prim<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+  } else {
+    if (f32) hipLaunchKernelGGL(k_prim32<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+    else hipLaunchKernelGGL(k_prim<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+  }
   return hipGetLastError();
 }
 

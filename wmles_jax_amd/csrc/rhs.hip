@@ -1,12 +1,13 @@
 // Right-hand side of the explicit step and the fused SSP-RK3 stage update.
 //
-//   k_prim     all ghosted cells: q -> rho u v w p T mu e_int          (gas.py:24-33)
 //   k_grad     cells [ng-1, ng+n] per axis: Green-Gauss grad of [u v w T], SGS mu_t,
 //              Ducros theta, mu_eff, k_eff                            (viscous.py:6-25,
 //                                                                       sgs.py, solver.py:78-85)
 //   k_wall     WallModel faces: wall flux, tau_w, q_w                 (bc.py:145-166),
 //              with the Newton model or the learned model of wallnet.h
-//   (the fused face-flux / divergence / stage kernel is in flux.hip)
+//   (the primitive kernels, q -> rho u v w p T mu e_int for all ghosted cells (gas.py:24-33),
+//   are in reduce.hip with the fused CFL partial; the fused face-flux / divergence / stage
+//   kernel is in flux.hip)
 #include "common.h"
 #include "wallnet.h"
 
@@ -17,49 +18,6 @@ __device__ __forceinline__ int find_block(const int* __restrict__ prefix, int nb
   int b = 0;
   while (b + 1 < nblk && prefix[b + 1] <= wg) ++b;
   return b;
-}
-
-// ------------------------------------------------------------------------------ prim
-__global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
-                                              const int* __restrict__ wg_prefix, int nblk,
-                                              Ptrs q, GasC gas) {
-  const int b = find_block(wg_prefix, nblk, blockIdx.x);
-  const BlockDev& B = blks[b];
-  long long c = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  const long long cells = B.cells;
-  if (c >= cells) return;
-  const double* qb = q.p[b];
-  Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
-                        qb[4 * cells + c]);
-  double* P = B.P;
-  P[c] = w.r;
-  P[cells + c] = w.u;
-  P[2 * cells + c] = w.v;
-  P[3 * cells + c] = w.w;
-  P[4 * cells + c] = w.p;
-  P[5 * cells + c] = w.T;
-  P[6 * cells + c] = mu_suth(gas, w.T);
-  P[7 * cells + c] = w.p / ((gas.gamma - 1.0) * w.r);
-}
-
-// FP32-bulk record: prims evaluated in FP64, stored in FP32 with p, T and e_int
-// shifted by the block gauge so their small variations keep FP32 precision.
-__global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blks,
-                                                const int* __restrict__ wg_prefix, int nblk,
-                                                Ptrs q, NumDev nd) {
-  const int b = find_block(wg_prefix, nblk, blockIdx.x);
-  const BlockDev& B = blks[b];
-  long long c = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  const long long cells = B.cells;
-  if (c >= cells) return;
-  const GasC& gas = nd.gas;
-  const double* qb = q.p[b];
-  Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
-                        qb[4 * cells + c]);
-  st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - nd.T_gauge)));
-  st4(B.Pf + 4 * ((size_t)cells + c),
-      make_float4((float)w.r, (float)(w.p - nd.p_ref), (float)mu_suth(gas, w.T),
-                  (float)(w.p / ((gas.gamma - 1.0) * w.r) - nd.e_ref)));
 }
 
 // ------------------------------------------------------------------------------ grad
@@ -458,16 +416,6 @@ hipError_t launch_wm_eval(const NumDev& nd, int mode, long long n, const double*
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_wm_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nd, mode, n, a,
                      b, c, d, T_wall, o0, o1);
-  return hipGetLastError();
-}
-
-hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
-                       const NumDev& nd, hipStream_t s) {
-  if (nwg <= 0) return hipSuccess;
-  if (nd.prec == WMX_PREC_FP32_BULK)
-    hipLaunchKernelGGL(k_prim32, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd);
-  else
-    hipLaunchKernelGGL(k_prim, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd.gas);
   return hipGetLastError();
 }
 
