@@ -2,13 +2,16 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form both|fp64|fp32]
                     [--workload channel256|ramp512|ensemble16|wm512] [--no-cpu-baseline]
+                    [--graph] [--breakdown]
 
 Default workload (channel256): compressible wall-modelled channel, Ma_b 0.3, Re_b 2e4,
 one block of 256 x 128 x 128 cells (x, y, z), tanh-clustered y, Hybrid(KEP(4),
 MUSCL-HLLC) with the Ducros switch, Vreman SGS, isothermal equilibrium wall model on
 both walls, periodic x / z, constant mass flux, CFL 0.8, seeded synthetic initial state
 (bulk profile + 5% Gaussian velocity noise).  A step is the reference's driver step:
-max_dt + advance (3 RHS + 3 stage updates + constraint), dt kept on the device.  The
+max_dt + advance (3 RHS + 3 stage updates + constraint), dt kept on the device, run as
+Solver.step (the CFL min is taken in the first stage's primitive pass, the same value
+as max_dt; eager launches, --graph replays captured hipGraphs instead).  The
 timed region holds exactly K steps bracketed by device synchronisations.  Both
 arithmetic forms run; the FP32-bulk form is the headline value, the FP64 form and the
 FP32-bulk errors (u_tau, mean u(y), q_w against FP64 after the same steps) are reported
@@ -17,7 +20,7 @@ beside it.
 Other workloads (one JSON line each):
   ramp512     smooth-ramp curvilinear block 512 x 128 x 64, inflow + sponge, FP32-bulk
   ensemble16  16 uneven multiblock channel cases in one plan (one launch per kernel
-              class and stage), graph-captured step, autotuned tiling, against the
+              class and stage), autotuned tiling, against the
               same cases stepped one after another
   wm512       channel 512 x 256 x 256 at Re_b 1e5 with the learned wall model (trained
               here on the GPU) against the Newton model: wall-kernel time, u_tau error
@@ -84,7 +87,7 @@ def traffic_from_profile(workload):
     return rec.get("hbm_bytes_per_launch")
 
 
-def run_form(wmx, form, size, steps, warmup, breakdown):
+def run_form(wmx, form, size, steps, warmup, breakdown, graph=False):
     """Time `steps` steps of one arithmetic form; returns timings, final state, wall stats."""
     import torch
     import cases
@@ -95,18 +98,26 @@ def run_form(wmx, form, size, steps, warmup, breakdown):
     pr = spec["props"]
     Q = s.init_state(lambda xc: cases.channel_prim(xc, pr, 0))
     for _ in range(warmup):
-        Q = s.advance(Q, s.max_dt(Q))
+        Q = s.step(Q, graph=graph)[0]
     torch.cuda.synchronize()
     s.profile(s.KERNEL_CLASSES if breakdown else ("flux",))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        dt = s.max_dt(Q)
-        Q = s.advance(Q, dt)
+        Q, dt = s.step(Q, graph=graph)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     prof = s.profile_read()
     s.profile(())
+    if graph:
+        # replayed launches are not event-timed: the kernel times come from eager steps
+        s.profile(s.KERNEL_CLASSES if breakdown else ("flux",))
+        Qp = Q
+        for _ in range(5):
+            Qp = s.step(Qp, graph=False)[0]
+        torch.cuda.synchronize()
+        prof = s.profile_read()
+        s.profile(())
     return {"elapsed": el, "prof": prof, "Q": Q, "mesh": mesh, "solver": s, "props": pr}
 
 
@@ -148,7 +159,7 @@ def bench_channel(args, wmx):
     nx, ny, nz = args.size
     cells = nx * ny * nz
     forms = ["fp64", "fp32"] if args.form == "both" else [args.form]
-    res = {f: run_form(wmx, f, args.size, args.steps, args.warmup, args.breakdown) for f in forms}
+    res = {f: run_form(wmx, f, args.size, args.steps, args.warmup, args.breakdown, args.graph) for f in forms}
     head = "fp32" if "fp32" in res else "fp64"
     r = res[head]
     el = r["elapsed"]
@@ -158,7 +169,9 @@ def bench_channel(args, wmx):
     workload = f"channel {nx}x{ny}x{nz} Hybrid(KEP4+MUSCL-HLLC) Vreman WM {head}"
     rec = _record(args, value, el * 1e3 / args.steps, "f32-bulk+f64" if head == "fp32" else "f64",
                   workload, {"cells": cells, "blocks": 1, "ng": 3,
-                             "step": "max_dt + advance (SSP-RK3, 3 RHS, mass-flux constraint)",
+                             "step": "Solver.step = max_dt + advance (SSP-RK3, 3 RHS, mass-flux "
+                                     "constraint; the CFL min taken in the first stage's primitive pass)",
+                             "graph": bool(args.graph),
                              "form": head},
                   _roofline(flux_ms / flux_n / 1e3, b_launch, workload))
     rec["step_hbm_frac"] = round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
@@ -238,7 +251,7 @@ def bench_ensemble(args, wmx):
     best, timings = autotune.tune(ens, Qs0, steps=3, use_cache=False)
 
     def ens_step(Qs):
-        return ens.step(Qs, graph=True)[0]
+        return ens.step(Qs, graph=args.graph)[0]
 
     el_e, Qe = _timed_steps(ens_step, Qs0, args.steps, args.warmup)
     # flux launch time from events on eager (non-replayed) steps of the same plan
@@ -248,14 +261,14 @@ def bench_ensemble(args, wmx):
         Qp = ens.step(Qp, graph=False)[0]
     prof = ens.profile_read()
     ens.profile(())
-    # the same cases stepped one after another (graph-captured per case), with the
+    # the same cases stepped one after another (the same step form per case), with the
     # ensemble's tiling and FP32-bulk gauge so the results can be compared bitwise
     for s in solvers:
         s.set_tiling(*best)
         s.set_gauge(*ens._gauge)
 
     def seq_step(Qs):
-        return [s.step(Q, graph=True)[0] for s, Q in zip(solvers, Qs)]
+        return [s.step(Q, graph=args.graph)[0] for s, Q in zip(solvers, Qs)]
 
     el_s, Qseq = _timed_steps(seq_step, Qs0, args.steps, args.warmup)
     same = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qseq) for a, b in zip(Qa, Qb))
@@ -296,9 +309,17 @@ def bench_wm(args, wmx):
         s.set_wall_net(net)
         st0 = s.wall_stats(Q0)
         s.profile(("wall", "flux"))
-        el, Q = _timed_steps(lambda Q: s.advance(Q, s.max_dt(Q)), Q0, args.steps, args.warmup)
+        el, Q = _timed_steps(lambda Q: s.step(Q, graph=args.graph)[0], Q0, args.steps, args.warmup)
         prof = s.profile_read()
         s.profile(())
+        if args.graph:                     # replayed launches are not event-timed
+            s.profile(("wall", "flux"))
+            Qp = Q
+            for _ in range(5):
+                Qp = s.step(Qp, graph=False)[0]
+            torch.cuda.synchronize()
+            prof = s.profile_read()
+            s.profile(())
         return el, Q, prof, st0
 
     el_n, Qn, prof_n, st_n = run(None)
@@ -362,6 +383,8 @@ def main():
     ap.add_argument("--size", type=int, nargs=3, default=(256, 128, 128))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="time every kernel class (extra events)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay captured hipGraphs for Solver.step (eager launches otherwise)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs the step on a single MI355X "

@@ -154,6 +154,12 @@ class EnsembleSolver:
         """max_dt + advance for every case; graph-captured replay when pointers repeat."""
         flat = self._flat(Qs)
         self._ensure_gauge(Qs)
+        if not graph:                 # eager: the caller's stream, no side-stream handshake
+            out = [torch.empty_like(q) for q in flat]
+            dt = torch.empty(max(16, self.n_cases), dtype=torch.float64, device=self._dev)
+            _lib.check(_lib.lib.wmx_step(self._plan, _lib.ptr_array(flat), _lib.ptr_array(out),
+                                         C.c_void_p(dt.data_ptr()), 0.0, 0, _lib.stream_ptr()))
+            return self._split(out), dt[:self.n_cases]
         cur = torch.cuda.current_stream(self._dev)
         self._stream.wait_stream(cur)
         with torch.cuda.stream(self._stream):

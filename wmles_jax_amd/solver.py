@@ -312,6 +312,16 @@ class Solver:
         `Q, dt = solver.step(Q)` loop alternates two buffers).  Returns (Q_new, dt)."""
         Q = self._as_state(Q)
         self._ensure_gauge(Q)
+        if not graph:                 # eager: the caller's stream, no side-stream handshake
+            out = tuple(torch.empty_like(q) for q in Q)
+            dt = torch.empty(16, dtype=torch.float64, device=self._dev)
+            _lib.check(_lib.lib.wmx_step(self._plan, _lib.ptr_array(Q), _lib.ptr_array(out),
+                                         C.c_void_p(dt.data_ptr()), float(time), 0, _lib.stream_ptr()))
+            for c in self._py_constraints:
+                out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
+            return out, dt[0]
+        # capture needs a stream other than the legacy default one: a side stream that
+        # keeps the dt buffer the graph was captured with
         if not hasattr(self, "_stream"):
             self._stream = torch.cuda.Stream(device=self._dev)
             self._dt_dev = torch.empty(16, dtype=torch.float64, device=self._dev)
