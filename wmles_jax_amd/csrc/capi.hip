@@ -103,6 +103,8 @@ struct wmx_plan {
   // the same per-case workgroup lists for the all-cells layout (fused stage-1 prim + dt)
   std::vector<int> case_wg_all, case_off_all;
   int *d_case_wg_all = nullptr, *d_case_off_all = nullptr;
+  // a list that is the identity (blocks ordered by case) is passed as nullptr
+  bool case_wg_ident = false, case_wg_all_ident = false;
   double* d_case_target = nullptr;
   // inflows
   std::vector<InflowDev> inflows;
@@ -432,8 +434,8 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
   }));
   if (dt_out)
     WMX_CHECK(timed(p, K_DT, s, [&] {
-      return launch_case_final(0, p->n_cases, p->d_case_wg_all, p->d_case_off_all, p->d_part, nullptr,
-                               dt_out, s);
+      return launch_case_final(0, p->n_cases, p->case_wg_all_ident ? nullptr : p->d_case_wg_all,
+                               p->d_case_off_all, p->d_part, nullptr, dt_out, s);
     }));
   WMX_CHECK(timed(p, K_GRAD, s, [&] { return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s); }));
   WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
@@ -605,7 +607,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     c_all.push_back(B.cells);
     const int run = grad_run(nd.prec == WMX_PREC_FP32_BULK ? sizeof(float) : sizeof(double));
     c_box.push_back((long long)((B.n[0] + 2 + run - 1) / run) * (B.n[1] + 2) * (B.n[2] + 2));
-    c_int.push_back((long long)B.n[0] * B.n[1] * B.n[2]);
+    c_int.push_back(((long long)B.n[0] * B.n[1] * B.n[2] + INT_CELLS - 1) / INT_CELLS);   // thread slots
     c_shell.push_back(B.shell_cells);
   }
   p->pre_all = prefix(c_all);
@@ -626,6 +628,13 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
         for (int w = p->pre_all[b]; w < p->pre_all[b + 1]; ++w) p->case_wg_all.push_back(w);
     p->case_off_all.push_back((int)p->case_wg_all.size());
   }
+  auto ident = [](const std::vector<int>& v) {
+    for (size_t i = 0; i < v.size(); ++i)
+      if (v[i] != (int)i) return false;
+    return true;
+  };
+  p->case_wg_ident = ident(p->case_wg);
+  p->case_wg_all_ident = ident(p->case_wg_all);
   // workspace layout
   size_t off = 0;
   p->off_wall.assign(nblocks, std::vector<long long>(6, -1));
@@ -922,7 +931,7 @@ int wmx_mass_flux(wmx_plan* p, double* const* q, void* stream) {
     hipError_t e = launch_mf_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq,
                                      p->d_case_dir, p->d_part, s);
     if (e == hipSuccess)
-      e = launch_case_final(1, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part,
+      e = launch_case_final(1, p->n_cases, p->case_wg_ident ? nullptr : p->d_case_wg, p->d_case_off, p->d_part,
                             p->d_case_target, p->d_delta, s);
     if (e == hipSuccess)
       e = launch_mf_apply(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), pq, p->d_case_dir,
@@ -973,7 +982,8 @@ int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
     hipError_t e = launch_dt_partial(p->d_blks, p->d_pre_int, p->nb, nwg_of(p->pre_int), pq, p->nd,
                                      p->d_part, s);
     if (e == hipSuccess)
-      e = launch_case_final(0, p->n_cases, p->d_case_wg, p->d_case_off, p->d_part, nullptr, d_dt, s);
+      e = launch_case_final(0, p->n_cases, p->case_wg_ident ? nullptr : p->d_case_wg, p->d_case_off,
+                            p->d_part, nullptr, d_dt, s);
     return e;
   }));
   return 0;

@@ -127,6 +127,7 @@ hipError_t launch_shell_finish(int mode, const BlockDev* d_blks, const int* d_pr
                                Ptrs q0, Ptrs out, const double* d_dt, double dt, hipStream_t s);
 hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
                              const NumDev& nd, double* d_part, hipStream_t s);
+constexpr int INT_CELLS = 4;      // interior cells per thread of the dt / mass-flux partials
 constexpr int NSTAT = 12;          // plane statistics: rho u v w uu vv ww uv uw vw T p
 constexpr int PLANE_CHUNKS = 16;   // workgroups per j plane in the plane statistics
 hipError_t launch_plane_stats(const BlockDev* d_blks, const BlockDev& hb, int blk, Ptrs q,

@@ -98,19 +98,23 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
   const int wg = xcd_span(blockIdx.x, gridDim.x);
   const int b = find_block_r(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
-  const unsigned l = (unsigned)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  // INT_CELLS interior cells per thread, blockDim.x apart
+  const unsigned base = (unsigned)(wg - wg_prefix[b]) * (INT_CELLS * blockDim.x) + threadIdx.x;
   const unsigned nint = (unsigned)B.n[0] * B.n[1] * B.n[2];
   double dtc = __builtin_huge_val();
-  if (l < nint) {
-    int I, J, K;
-    const unsigned c = interior_cell(B, l, &I, &J, &K);
-    const unsigned cells = (unsigned)B.cells;
-    const double* qb = q.p[b];
-    double qv[5];
+  for (int u = 0; u < INT_CELLS; ++u) {
+    const unsigned l = base + (unsigned)u * blockDim.x;
+    if (l < nint) {
+      int I, J, K;
+      const unsigned c = interior_cell(B, l, &I, &J, &K);
+      const unsigned cells = (unsigned)B.cells;
+      const double* qb = q.p[b];
+      double qv[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) qv[v] = ldu(qb + v * cells, c);
-    const Prim w = cons_to_prim(nd.gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
-    dtc = cell_dt(B, nd, c, I, J, K, w);
+      for (int v = 0; v < 5; ++v) qv[v] = ldu(qb + v * cells, c);
+      const Prim w = cons_to_prim(nd.gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
+      dtc = nmin(dtc, cell_dt(B, nd, c, I, J, K, w));
+    }
   }
   const double r = block_reduce<true>(dtc);
   if (threadIdx.x == 0) part[wg] = r;
@@ -206,16 +210,30 @@ __global__ __launch_bounds__(256) void k_mf_partial(const BlockDev* __restrict__
                                                     double* __restrict__ part) {
   const int b = find_block_r(wg_prefix, nblk, blockIdx.x);
   const BlockDev& B = blks[b];
-  const unsigned l = (unsigned)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
+  // INT_CELLS interior cells per thread, blockDim.x apart, added in that order
+  const unsigned base = (unsigned)(blockIdx.x - wg_prefix[b]) * (INT_CELLS * blockDim.x) + threadIdx.x;
   const unsigned nint = (unsigned)B.n[0] * B.n[1] * B.n[2];
   const int dir = case_dir[B.case_id];
   double num = 0.0, den = 0.0;
-  if (l < nint && dir >= 0) {
-    int I, J, K;
-    const unsigned c = interior_cell(B, l, &I, &J, &K);
-    const double vol = ldu(B.vol, c);
-    num = ldu(q.p[b] + (unsigned)(1 + dir) * (unsigned)B.cells, c) * vol;
-    den = vol;
+  if (dir >= 0) {
+    double vv[INT_CELLS], qm[INT_CELLS];
+#pragma unroll
+    for (int u = 0; u < INT_CELLS; ++u) {
+      const unsigned l = base + (unsigned)u * blockDim.x;
+      vv[u] = 0.0;
+      qm[u] = 0.0;
+      if (l < nint) {
+        int I, J, K;
+        const unsigned c = interior_cell(B, l, &I, &J, &K);
+        vv[u] = ldu(B.vol, c);
+        qm[u] = ldu(q.p[b] + (unsigned)(1 + dir) * (unsigned)B.cells, c);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < INT_CELLS; ++u) {
+      num = num + qm[u] * vv[u];
+      den = den + vv[u];
+    }
   }
   const double rn = block_reduce<false>(num);
   const double rd = block_reduce<false>(den);
@@ -238,18 +256,20 @@ __global__ __launch_bounds__(1024) void k_case_final(const int* __restrict__ cas
   const int lo = case_wg_off[cs], hi = case_wg_off[cs + 1];
   double a = MODE == 0 ? __builtin_huge_val() : 0.0, b = 0.0;
   const int stride = blockDim.x;
-  for (int t0 = lo + threadIdx.x; t0 < hi; t0 += 4 * stride) {
-    double va[4], vb[4];
+  // case_wg == nullptr: the list is the identity (blocks ordered by case), so the partial
+  // index is the list position and all eight loads of an iteration are independent
+  for (int t0 = lo + threadIdx.x; t0 < hi; t0 += 8 * stride) {
+    double va[8], vb[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int t = t0 + u * stride;
       const bool ok = t < hi;
-      const int wg = ok ? case_wg[t] : 0;
+      const int wg = ok ? (case_wg ? case_wg[t] : t) : 0;
       va[u] = ok ? (MODE == 0 ? part[wg] : part[2 * wg]) : (MODE == 0 ? __builtin_huge_val() : 0.0);
       vb[u] = (ok && MODE == 1) ? part[2 * wg + 1] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       if (MODE == 0) a = nmin(a, va[u]);
       else { a += va[u]; b += vb[u]; }
     }
