@@ -150,8 +150,9 @@ class EnsembleSolver:
                                         C.c_void_p(d.data_ptr()), float(time), _lib.stream_ptr()))
         return self._split(out)
 
-    def step(self, Qs, graph=True):
-        """max_dt + advance for every case; graph-captured replay when pointers repeat."""
+    def step(self, Qs, graph=False):
+        """max_dt + advance for every case (caller's stream); graph=True: captured-graph
+        replay when the state pointers repeat."""
         flat = self._flat(Qs)
         self._ensure_gauge(Qs)
         if not graph:                 # eager: the caller's stream, no side-stream handshake

@@ -305,11 +305,13 @@ class Solver:
             out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
         return out
 
-    def step(self, Q, time=0.0, graph=True):
+    def step(self, Q, time=0.0, graph=False):
         """The reference driver's step, max_dt then advance (examples/channel_wmles.py:
-        124-125), with dt kept on the device.  graph=True replays the whole launch
-        sequence as one captured hipGraph when the state pointers repeat (the usual
-        `Q, dt = solver.step(Q)` loop alternates two buffers).  Returns (Q_new, dt)."""
+        124-125), with dt kept on the device (the CFL min is taken in the first stage's
+        primitive pass; same value as max_dt).  Launches go to the caller's stream;
+        graph=True instead replays the whole launch sequence as one captured hipGraph
+        when the state pointers repeat (the usual `Q, dt = solver.step(Q)` loop alternates
+        two buffers).  Returns (Q_new, dt)."""
         Q = self._as_state(Q)
         self._ensure_gauge(Q)
         if not graph:                 # eager: the caller's stream, no side-stream handshake
