@@ -108,3 +108,56 @@ def test_plane_stats_match_the_oracle():
     assert abs(ut - ut_o) <= 1e-14 * ut_o
     for k in out_o:
         np.testing.assert_allclose(out[k], out_o[k], rtol=1e-10, atol=1e-12 * np.abs(out_o[k]).max())
+
+
+def test_blocks_interleaved_across_cases():
+    """The same two-case plan with its blocks interleaved across the cases (per-case
+    workgroup lists that are not the identity: the general path of the per-case
+    reductions) gives the per-case dt and the mass-flux constraint bit for bit."""
+    import ctypes as C
+    from wmles_jax_amd import _lib
+    specs = [(2, 32, 16, 16, 2e4, 0, None), (3, 48, 16, 8, 8e3, 1, [0, 10, 30, 48])]
+    solvers, fns = [], []
+    for nb, nx, ny, nz, re, seed, sp in specs:
+        s, fn = _channel(nb, nx, ny, nz, re, seed, sp)
+        solvers.append(s)
+        fns.append(fn)
+    ens = wmx.EnsembleSolver(solvers)
+    Qs = ens.init_state(fns)
+    flat = ens._flat(Qs)
+    nblk = len(flat)
+    assert nblk == 5
+    perm = [0, 2, 1, 3, 4]                     # plan position -> case-ordered block
+    inv = {old: new for new, old in enumerate(perm)}
+    descs = (_lib.BlockDesc * nblk)()
+    for new, old in enumerate(perm):
+        d = _lib.BlockDesc()
+        C.memmove(C.byref(d), C.byref(ens._descs[old]), C.sizeof(_lib.BlockDesc))
+        for f in range(6):
+            if d.faces[f].kind == _lib.FACE_INTERFACE:
+                d.faces[f].nb = inv[d.faces[f].nb]
+        descs[new] = d
+    plan = C.c_void_p()
+    _lib.check(_lib.lib.wmx_plan_create(nblk, descs, C.byref(ens._nd), 0, None, C.byref(plan)))
+    try:
+        nbytes = C.c_size_t()
+        _lib.check(_lib.lib.wmx_plan_workspace_bytes(plan, C.byref(nbytes)))
+        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=flat[0].device)
+        _lib.check(_lib.lib.wmx_plan_bind_workspace(plan, C.c_void_p(ws.data_ptr()), nbytes.value))
+        dt_ref = ens.max_dt(Qs)
+        out = torch.empty(16, dtype=torch.float64, device=flat[0].device)
+        _lib.check(_lib.lib.wmx_max_dt(plan, _lib.ptr_array([flat[o] for o in perm]),
+                                       C.c_void_p(out.data_ptr()), _lib.stream_ptr()))
+        torch.cuda.synchronize()
+        assert torch.equal(out[:2], dt_ref)
+        A = [q.clone() for q in flat]
+        B = [q.clone() for q in flat]
+        _lib.check(_lib.lib.wmx_mass_flux(ens._plan, _lib.ptr_array(A), _lib.stream_ptr()))
+        _lib.check(_lib.lib.wmx_mass_flux(plan, _lib.ptr_array([B[o] for o in perm]), _lib.stream_ptr()))
+        torch.cuda.synchronize()
+        assert any(not torch.equal(a, q) for a, q in zip(A, flat))     # the constraint acted
+        for a, b in zip(A, B):
+            assert torch.equal(a, b)
+    finally:
+        torch.cuda.synchronize()
+        _lib.lib.wmx_plan_destroy(plan)
