@@ -942,10 +942,11 @@ int wmx_mass_flux(wmx_plan* p, double* const* q, void* stream) {
 }
 
 namespace {
-// fuse_dt: d_dt is computed here, from q0, by the stage-1 primitive launch (the fill before
-// it changes ghosts only, and reads d_dt only for inflow faces, which never fuse)
-int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt, double* d_dt,
-                 double time, void* stream, bool fuse_dt) {
+// dt_out != nullptr (then dt_out == d_dt): the step's dt is computed here, from q0, by the
+// stage-1 primitive launch (the fill before it changes ghosts only, and reads d_dt only
+// for inflow faces, which never fuse)
+int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
+                 const double* d_dt, double* dt_out, double time, void* stream) {
   for (int b = 0; b < p->nb; ++b)
     if (q0[b] == qout[b]) return fail("qout must not alias q0");
   hipStream_t s = as_stream(stream);
@@ -954,7 +955,7 @@ int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
   WMX_CHECK(timed(p, K_SHELL, s, [&] { return launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s); }));
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
-  if (run_cells(p, P0, time, d_dt, 0.0, s, fuse_dt ? d_dt : nullptr)) return 1;
+  if (run_cells(p, P0, time, d_dt, 0.0, s, dt_out)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
@@ -970,7 +971,7 @@ int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
 int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, const double* d_dt,
                 double time, void* stream) {
   if (check_plan(p)) return 1;
-  return advance_impl(p, q0, qout, dt, const_cast<double*>(d_dt), time, stream, false);
+  return advance_impl(p, q0, qout, dt, d_dt, nullptr, time, stream);
 }
 
 int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
@@ -1034,7 +1035,7 @@ int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, 
     if (wmx_max_dt(p, q0, d_dt, stream)) return 1;
     return wmx_advance(p, q0, qout, 0.0, d_dt, time, stream);
   }
-  if (!use_graph || s == nullptr) return advance_impl(p, q0, qout, 0.0, d_dt, time, stream, true);
+  if (!use_graph || s == nullptr) return advance_impl(p, q0, qout, 0.0, d_dt, d_dt, time, stream);
   for (auto& g : p->graphs) {
     bool same = g.d_dt == d_dt && g.time == time;
     for (int b = 0; same && b < p->nb; ++b) same = g.q0[b] == q0[b] && g.qout[b] == qout[b];
@@ -1046,7 +1047,7 @@ int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, 
   hipGraph_t graph;
   WMX_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   p->capturing = true;
-  int rc = advance_impl(p, q0, qout, 0.0, d_dt, time, stream, true);
+  int rc = advance_impl(p, q0, qout, 0.0, d_dt, d_dt, time, stream);
   p->capturing = false;
   hipError_t e = hipStreamEndCapture(s, &graph);
   if (rc) return 1;
