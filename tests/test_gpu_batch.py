@@ -161,3 +161,20 @@ def test_blocks_interleaved_across_cases():
     finally:
         torch.cuda.synchronize()
         _lib.lib.wmx_plan_destroy(plan)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_step_with_inflow_equals_max_dt_plus_advance(precision):
+    """Time-dependent inflow: Solver.step takes the separate max_dt + advance path (the
+    fill reads dt), and its result equals those two calls bit for bit."""
+    spec, arrays, prim = cases.ramp_spec(wmx, 48, 24, 16, n_modes=8)
+    mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": precision})
+    Q0 = s.init_state(prim)
+    for t in (0.0, 1.5e-6):
+        A, dta = s.step(Q0, time=t, graph=False)
+        dtb = s.max_dt(Q0, t)
+        B = s.advance(Q0, dtb, t)
+        torch.cuda.synchronize()
+        assert float(dta) == float(dtb)
+        for a, b in zip(A, B):
+            assert torch.equal(a, b)
