@@ -214,9 +214,7 @@ def bench_ramp(args, wmx):
     t = [0.0]
 
     def step(Q):
-        dt = s.max_dt(Q, t[0])
-        Q = s.advance(Q, dt, t[0])
-        return Q
+        return s.step(Q, time=t[0], graph=False)[0]
 
     s.profile(("flux",))
     el, Q = _timed_steps(step, Q, args.steps, args.warmup)

@@ -207,8 +207,8 @@ int wmx_advance(wmx_plan* plan, double* const* q0, double* const* qout, double d
                 const double* d_dt, double time, void* stream);
 /* The reference driver's step (examples/channel_wmles.py:124-125: max_dt then advance)
    with dt kept on the device: d_dt[case] <- CFL dt of q0, then qout = advance(q0).
-   Without time-dependent inflow the dt partials come from the first stage's primitive
-   launch (same values as wmx_max_dt, one pass over q0 fewer).
+   The dt partials come from the first stage's primitive launch (same values as
+   wmx_max_dt, one pass over q0 fewer).
    use_graph != 0 (and a non-default stream, and no time-dependent inflow): the launch
    sequence is captured once per (q0, qout, d_dt, time) into a hipGraph (at most 4
    kept per plan) and replayed on later calls with the same pointers. */

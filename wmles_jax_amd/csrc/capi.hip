@@ -943,8 +943,8 @@ int wmx_mass_flux(wmx_plan* p, double* const* q, void* stream) {
 
 namespace {
 // dt_out != nullptr (then dt_out == d_dt): the step's dt is computed here, from q0, by the
-// stage-1 primitive launch (the fill before it changes ghosts only, and reads d_dt only
-// for inflow faces, which never fuse)
+// stage-1 primitive launch.  The fill before it changes ghosts only, and stage 1's time is
+// `time` itself (solver.py:125-129), so that fill takes no dt.
 int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
                  const double* d_dt, double* dt_out, double time, void* stream) {
   for (int b = 0; b < p->nb; ++b)
@@ -955,7 +955,7 @@ int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
   WMX_CHECK(timed(p, K_SHELL, s, [&] { return launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s); }));
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
-  if (run_cells(p, P0, time, d_dt, 0.0, s, dt_out)) return 1;
+  if (run_cells(p, P0, time, nullptr, 0.0, s, dt_out)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
@@ -1030,12 +1030,10 @@ int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, 
   for (auto& ph : p->phase_inflow) has_inflow |= !ph.empty();
   for (int b = 0; b < p->nb; ++b)
     if (q0[b] == qout[b]) return fail("qout must not alias q0");
-  // without inflow faces the step's dt comes from its stage-1 primitive launch
-  if (has_inflow) {
-    if (wmx_max_dt(p, q0, d_dt, stream)) return 1;
-    return wmx_advance(p, q0, qout, 0.0, d_dt, time, stream);
-  }
-  if (!use_graph || s == nullptr) return advance_impl(p, q0, qout, 0.0, d_dt, d_dt, time, stream);
+  // the step's dt comes from its stage-1 primitive launch; time-dependent inflow plans
+  // run eagerly (a captured graph would hold one time)
+  if (!use_graph || has_inflow || s == nullptr)
+    return advance_impl(p, q0, qout, 0.0, d_dt, d_dt, time, stream);
   for (auto& g : p->graphs) {
     bool same = g.d_dt == d_dt && g.time == time;
     for (int b = 0; same && b < p->nb; ++b) same = g.q0[b] == q0[b] && g.qout[b] == qout[b];
