@@ -73,6 +73,8 @@ struct wmx_plan {
   std::vector<int> phase_max;
   GhostOp* d_ops = nullptr;
   std::vector<std::vector<int>> phase_inflow;  // block ids of inflow fills per phase
+  bool fuse_last = false;   // the last phase is k-face fills only: a step's primitive launches
+                            // evaluate those ghosts (BlockDev::zfuse) and skip its fill launch
   // wall faces
   std::vector<WallOp> wall;
   WallOp* d_wall = nullptr;
@@ -405,8 +407,10 @@ Ptrs make_ptrs(const wmx_plan* p, double* const* q) {
   return r;
 }
 
-int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s) {
-  const int nph = (int)p->phase_max.size();
+// zfuse: the last phase is left to the primitive launch (plans with fuse_last)
+int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s,
+             bool zfuse = false) {
+  const int nph = (int)p->phase_max.size() - (zfuse && p->fuse_last ? 1 : 0);
   for (int ph = 0; ph < nph; ++ph) {
     const int o = p->phase_off[ph], n = p->phase_off[ph + 1] - o;
     WMX_CHECK(timed(p, K_FILL, s, [&] { return launch_fill_cells(p->d_blks, p->d_ops + o, n, p->phase_max[ph], q, p->nd.gas, s); }));
@@ -425,12 +429,14 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
 // ghost fill + cell-level fields of one RHS evaluation
 // dt_out != nullptr: the primitive launch also takes the CFL partials of the (unchanged)
 // interior state and the per-case minimum goes to dt_out before the gradients
+// zfuse: a step's stage (the sweep's last phase evaluated by the primitive launch)
 int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s,
-              double* dt_out = nullptr) {
-  if (run_fill(p, q, time, d_dt, coef, s)) return 1;
+              double* dt_out = nullptr, bool zfuse = false) {
+  zfuse = zfuse && p->fuse_last;
+  if (run_fill(p, q, time, d_dt, coef, s, zfuse)) return 1;
   WMX_CHECK(timed(p, K_PRIM, s, [&] {
     return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd,
-                       dt_out ? p->d_part : nullptr, s);
+                       dt_out ? p->d_part : nullptr, zfuse ? 1 : 0, s);
   }));
   if (dt_out)
     WMX_CHECK(timed(p, K_DT, s, [&] {
@@ -584,6 +590,23 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   }
   p->passes = effective_passes(*p, nm->bc_passes);
   build_phases(*p, p->passes);
+  // The last phase's values are read by nothing else in a step but the primitive kernel
+  // (the later kernels read the primitive records or interior states; the step's output
+  // ghosts come from the saved shell), so when it holds only k-face, non-inflow fills the
+  // primitive kernels of the step's stages evaluate them in place of its launch.
+  {
+    const int nph = (int)p->phase_max.size();
+    bool ok = nph > 0;
+    if (ok)
+      for (int o = p->phase_off[nph - 1]; o < p->phase_off[nph]; ++o) {
+        const GhostOp& op = p->ops[o];
+        if (op.axis != 2 || p->hb[op.blk].faces[4 + op.side].kind == WMX_BC_INFLOW) ok = false;
+      }
+    p->fuse_last = ok;
+    if (ok)
+      for (int o = p->phase_off[nph - 1]; o < p->phase_off[nph]; ++o)
+        p->hb[p->ops[o].blk].zfuse |= 1 << p->ops[o].side;
+  }
   // wall faces
   for (int b = 0; b < nblocks; ++b)
     for (int f = 0; f < 6; ++f)
@@ -955,11 +978,11 @@ int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
   WMX_CHECK(timed(p, K_SHELL, s, [&] { return launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s); }));
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
-  if (run_cells(p, P0, time, nullptr, 0.0, s, dt_out)) return 1;
+  if (run_cells(p, P0, time, nullptr, 0.0, s, dt_out, true)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
-  if (run_cells(p, PO, t1, d_dt, 1.0, s)) return 1;
+  if (run_cells(p, PO, t1, d_dt, 1.0, s, nullptr, true)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
-  if (run_cells(p, PO, t2, d_dt, 0.5, s)) return 1;
+  if (run_cells(p, PO, t2, d_dt, 0.5, s, nullptr, true)) return 1;
   WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(3, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
   WMX_CHECK(timed(p, K_SHELL, s, [&] {
     return launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s);

@@ -47,6 +47,8 @@ struct BlockDev {
   int* sponge_row;          // (n0, n1): 1 if some cell of the row has a sponge weight > 0
   double* shell;            // (5, shell cells) saved ghost values
   long long shell_cells;
+  int zfuse;                // bit s: the k ghosts of side s are the sweep's last phase, which
+                            // the primitive kernels of a step evaluate themselves
   wmx_face_desc faces[6];
 };
 
@@ -112,8 +114,10 @@ hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);
 // d_part != nullptr: the launch also writes the CFL partial of each workgroup (step fusion)
+// zfuse: the k ghosts flagged in BlockDev::zfuse are evaluated from the sweep (their fill
+// launch is skipped) instead of read from q
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
-                       const NumDev& nd, double* d_part, hipStream_t s);
+                       const NumDev& nd, double* d_part, int zfuse, hipStream_t s);
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
                        hipStream_t s);
 hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, int maxcount, Ptrs q,

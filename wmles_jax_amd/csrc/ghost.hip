@@ -3,44 +3,9 @@
 // Built with -ffp-contract=off: every value here is a copy or a short expression
 // evaluated in the reference's association order, so the ghost maps and the metric
 // terms agree with the reference bit for bit.
-#include "common.h"
+#include "ghostfill.h"
 
 namespace wmx {
-
-// Decode thread -> (m, c_dst[3]) for the ghost slab of (axis, side); dims = array extents.
-__device__ __forceinline__ void ghost_coords(int idx, int axis, int ng, const int* dims, int* m,
-                                             int* c) {
-  int t1, t2;
-  tangents(axis, &t1, &t2);
-  if (axis == 2) {                 // k faces: m fastest (contiguous along k)
-    *m = idx % ng;
-    int r = idx / ng;
-    c[1] = r % dims[1];
-    c[0] = r / dims[1];
-  } else {
-    c[t2] = idx % dims[t2];
-    int r = idx / dims[t2];
-    c[t1] = r % dims[t1];
-    *m = r / dims[t1];
-  }
-}
-
-// Cell / node exchange, reference wmles/mesh.py:125-155.  node != 0: node arrays
-// (extents N+1, interface plane shared, shift added).
-__device__ __forceinline__ void interface_src(const wmx_face_desc& f, int axis, int side, int m,
-                                              const int* c_dst, const int* dims_dst, int ng,
-                                              int nB, int node, int* c_src) {
-  int aB = f.src_axis, sB = f.src_side;
-  for (int t = 0; t < 3; ++t) {
-    if (t == axis) continue;
-    int v = f.flip[t] ? dims_dst[t] - 1 - c_dst[t] : c_dst[t];
-    c_src[f.mapping[t]] = v;
-  }
-  if (node)
-    c_src[aB] = sB == 1 ? nB + ng - 1 - m : ng + 1 + m;
-  else
-    c_src[aB] = sB == 1 ? nB + ng - 1 - m : ng + m;
-}
 
 __global__ void k_fill_nodes(const BlockDev* __restrict__ blks, const GhostOp* __restrict__ ops,
                              Ptrs X) {
@@ -79,7 +44,8 @@ __global__ void k_fill_nodes(const BlockDev* __restrict__ blks, const GhostOp* _
 }
 
 // Ghost-cell fill of one phase of the sequential sweep, reference solver.py:59-73,
-// mesh.py:125-155 (interfaces) and bc.py:14-143 (mirror fills).
+// mesh.py:125-155 (interfaces) and bc.py:14-143 (mirror fills); the per-cell value is
+// ghost_value (ghostfill.h).
 __global__ void k_fill_cells(const BlockDev* __restrict__ blks, const GhostOp* __restrict__ ops,
                              Ptrs q, GasC gas) {
   const GhostOp op = ops[blockIdx.y];
@@ -90,48 +56,12 @@ __global__ void k_fill_cells(const BlockDev* __restrict__ blks, const GhostOp* _
   int m, c[3];
   ghost_coords(idx, op.axis, ng, B.N, &m, c);
   const int a = op.axis, s = op.side, n = B.n[a];
-  const wmx_face_desc& f = B.faces[2 * a + s];
-  if (f.kind == WMX_BC_INFLOW) return;            // filled by k_inflow
   c[a] = s == 1 ? ng + n + m : ng - 1 - m;
+  double out[5];
+  if (!ghost_value(blks, op.blk, a, s, m, c, q, gas, out)) return;   // inflow: k_inflow
   double* qd = q.p[op.blk];
   const long long cells = B.cells;
   const long long di = cidx(B.N, c[0], c[1], c[2]);
-  if (f.kind == WMX_FACE_INTERFACE) {
-    const BlockDev& S = blks[f.nb];
-    int cs[3];
-    interface_src(f, a, s, m, c, B.N, ng, S.n[f.src_axis], 0, cs);
-    const double* qs = q.p[f.nb];
-    const long long si = cidx(S.N, cs[0], cs[1], cs[2]);
-    for (int v = 0; v < 5; ++v) qd[v * cells + di] = qs[v * S.cells + si];
-    return;
-  }
-  int ci[3] = {c[0], c[1], c[2]};
-  ci[a] = s == 0 ? ng + m : ng + n - 1 - m;
-  if (f.kind == WMX_BC_EXTRAPOLATE) ci[a] = s == 0 ? ng : ng + n - 1;
-  const long long ii = cidx(B.N, ci[0], ci[1], ci[2]);
-  double src[5], out[5];
-  for (int v = 0; v < 5; ++v) src[v] = qd[v * cells + ii];
-  if (f.kind == WMX_BC_EXTRAPOLATE) {
-    for (int v = 0; v < 5; ++v) out[v] = src[v];
-  } else if (f.kind == WMX_BC_DIRICHLET) {
-    prim_to_cons(gas, f.prim[0], f.prim[1], f.prim[2], f.prim[3], f.prim[4], out);
-  } else if (f.kind == WMX_BC_NOSLIP) {
-    bc_noslip(gas, src, f.T_wall, out);
-  } else {
-    // unit normal of the boundary face at this tangential position (bc.py:17-19)
-    int cf[3] = {c[0], c[1], c[2]};
-    cf[a] = s == 0 ? ng : ng + n;
-    const long long fi = fidx(B.N, a, cf[0], cf[1], cf[2]);
-    const long long fc = fcount(B.N, a);
-    const double* S = B.S[a];
-    double Sx = S[fi], Sy = S[fc + fi], Sz = S[2 * fc + fi];
-    double Sm = sqrt((Sx * Sx + Sy * Sy) + Sz * Sz);
-    double nx = Sx / Sm, ny = Sy / Sm, nz = Sz / Sm;
-    if (f.kind == WMX_BC_FARFIELD)
-      bc_farfield(gas, src, f.prim, nx, ny, nz, s, out);
-    else                                            // SLIP, WALLMODEL
-      bc_slip(gas, src, nx, ny, nz, out);
-  }
   for (int v = 0; v < 5; ++v) qd[v * cells + di] = out[v];
 }
 

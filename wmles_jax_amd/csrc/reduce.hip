@@ -5,7 +5,7 @@
 // (gas.py:24-33), whose stage-1 launch of a step can take the CFL partials, so the
 // separate dt kernel and the fused one evaluate the same code.  Built with
 // -ffp-contract=off.
-#include "common.h"
+#include "ghostfill.h"
 
 namespace wmx {
 
@@ -135,11 +135,34 @@ __device__ __forceinline__ bool interior_of(const BlockDev& B, unsigned c, int* 
          *K < ng + B.n[2];
 }
 
+// State of cell c: from q, or (zfuse) for a k ghost of a flagged side the value the sweep's
+// last phase would store there (ghost_value, the fill kernel's own code)
+__device__ __forceinline__ void prim_state(const BlockDev* __restrict__ blks, int b, long long c,
+                                           Ptrs q, const GasC& gas, int zfuse, double* qv) {
+  const BlockDev& B = blks[b];
+  if (zfuse && B.zfuse) {
+    const long long N2 = B.N[2];
+    const long long r = c / N2;
+    const int K = (int)(c - r * N2), ng = B.ng, n2 = B.n[2];
+    const int side = K < ng ? 0 : (K >= ng + n2 ? 1 : -1);
+    if (side >= 0 && ((B.zfuse >> side) & 1)) {
+      const int cc[3] = {(int)(r / B.N[1]), (int)(r % B.N[1]), K};
+      const int m = side == 0 ? ng - 1 - K : K - ng - n2;
+      if (ghost_value(blks, b, 2, side, m, cc, q, gas, qv)) return;
+    }
+  }
+  const double* qb = q.p[b];
+  const long long cells = B.cells;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) qv[v] = qb[v * cells + c];
+}
+
 // FP64 record (r, u, v, w, p, T, mu, e_int)
 template <bool DT>
 __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
                                               const int* __restrict__ wg_prefix, int nblk,
-                                              Ptrs q, NumDev nd, double* __restrict__ part) {
+                                              Ptrs q, NumDev nd, double* __restrict__ part,
+                                              int zfuse) {
   // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
   // stay in one L2
   const int wg = xcd_span(blockIdx.x, gridDim.x);
@@ -150,9 +173,9 @@ __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
   double dtc = __builtin_huge_val();
   if (c < cells) {
     const GasC& gas = nd.gas;
-    const double* qb = q.p[b];
-    const Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
-                                qb[4 * cells + c]);
+    double qv[5];
+    prim_state(blks, b, c, q, gas, zfuse, qv);
+    const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
     double* P = B.P;
     P[c] = w.r;
     P[cells + c] = w.u;
@@ -176,7 +199,8 @@ __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
 template <bool DT>
 __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blks,
                                                 const int* __restrict__ wg_prefix, int nblk,
-                                                Ptrs q, NumDev nd, double* __restrict__ part) {
+                                                Ptrs q, NumDev nd, double* __restrict__ part,
+                                                int zfuse) {
   // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
   // stay in one L2
   const int wg = xcd_span(blockIdx.x, gridDim.x);
@@ -187,9 +211,9 @@ __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blk
   double dtc = __builtin_huge_val();
   if (c < cells) {
     const GasC& gas = nd.gas;
-    const double* qb = q.p[b];
-    const Prim w = cons_to_prim(gas, qb[c], qb[cells + c], qb[2 * cells + c], qb[3 * cells + c],
-                                qb[4 * cells + c]);
+    double qv[5];
+    prim_state(blks, b, c, q, gas, zfuse, qv);
+    const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
     st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - nd.T_gauge)));
     st4(B.Pf + 4 * ((size_t)cells + c),
         make_float4((float)w.r, (float)(w.p - nd.p_ref), (float)mu_suth(gas, w.T),
@@ -446,15 +470,15 @@ hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk,
 }
 
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
-                       const NumDev& nd, double* d_part, hipStream_t s) {
+                       const NumDev& nd, double* d_part, int zfuse, hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   const bool f32 = nd.prec == WMX_PREC_FP32_BULK;
   if (d_part) {
-    if (f32) hipLaunchKernelGGL(k_prim32<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
-    else hipLaunchKernelGGL(k_prim<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+    if (f32) hipLaunchKernelGGL(k_prim32<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
+    else hipLaunchKernelGGL(k_prim<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
   } else {
-    if (f32) hipLaunchKernelGGL(k_prim32<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
-    else hipLaunchKernelGGL(k_prim<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part);
+    if (f32) hipLaunchKernelGGL(k_prim32<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
+    else hipLaunchKernelGGL(k_prim<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
   }
   return hipGetLastError();
 }
