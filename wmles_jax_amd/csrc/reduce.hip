@@ -141,12 +141,14 @@ __device__ __forceinline__ void prim_state(const BlockDev* __restrict__ blks, in
                                            Ptrs q, const GasC& gas, int zfuse, double* qv) {
   const BlockDev& B = blks[b];
   if (zfuse && B.zfuse) {
-    const long long N2 = B.N[2];
-    const long long r = c / N2;
-    const int K = (int)(c - r * N2), ng = B.ng, n2 = B.n[2];
+    // 32-bit decode (every block holds fewer than 2^32 cells); I, J only for the ghosts
+    const unsigned N2 = (unsigned)B.N[2];
+    const unsigned r = (unsigned)c / N2;
+    const int K = (int)((unsigned)c - r * N2), ng = B.ng, n2 = B.n[2];
     const int side = K < ng ? 0 : (K >= ng + n2 ? 1 : -1);
     if (side >= 0 && ((B.zfuse >> side) & 1)) {
-      const int cc[3] = {(int)(r / B.N[1]), (int)(r % B.N[1]), K};
+      const unsigned N1 = (unsigned)B.N[1];
+      const int cc[3] = {(int)(r / N1), (int)(r % N1), K};
       const int m = side == 0 ? ng - 1 - K : K - ng - n2;
       if (ghost_value(blks, b, 2, side, m, cc, q, gas, qv)) return;
     }
