@@ -443,8 +443,14 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
       return launch_case_final(0, p->n_cases, p->case_wg_all_ident ? nullptr : p->d_case_wg_all,
                                p->d_case_off_all, p->d_part, nullptr, dt_out, s);
     }));
-  WMX_CHECK(timed(p, K_GRAD, s, [&] { return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd, s); }));
-  WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
+  // FP32-bulk plans with the Newton model take the wall faces into the gradient launch
+  const bool wall_in_grad = p->nd.prec == WMX_PREC_FP32_BULK && !p->nd.net.w;
+  WMX_CHECK(timed(p, K_GRAD, s, [&] {
+    return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd,
+                       wall_in_grad ? p->d_wall : nullptr, (int)p->wall.size(), p->wall_max, q, s);
+  }));
+  if (!wall_in_grad)
+    WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
   if (p->nd.n_sponge > 0)
     for (int b = 0; b < p->nb; ++b)
       WMX_CHECK(timed(p, K_SPONGE, s, [&] {
