@@ -438,16 +438,26 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
     return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd,
                        dt_out ? p->d_part : nullptr, zfuse ? 1 : 0, s);
   }));
-  if (dt_out)
+  // FP32-bulk plans take the per-case dt mins and (Newton model) the wall faces into the
+  // gradient launch; the flux kernel after it is the first reader of both
+  const bool f32 = p->nd.prec == WMX_PREC_FP32_BULK;
+  const bool wall_in_grad = f32 && !p->nd.net.w;
+  GradDt gdt;
+  if (dt_out && f32) {
+    gdt.ncases = p->n_cases;
+    gdt.list = p->case_wg_all_ident ? nullptr : p->d_case_wg_all;
+    gdt.off = p->d_case_off_all;
+    gdt.part = p->d_part;
+    gdt.out = dt_out;
+  } else if (dt_out) {
     WMX_CHECK(timed(p, K_DT, s, [&] {
       return launch_case_final(0, p->n_cases, p->case_wg_all_ident ? nullptr : p->d_case_wg_all,
                                p->d_case_off_all, p->d_part, nullptr, dt_out, s);
     }));
-  // FP32-bulk plans with the Newton model take the wall faces into the gradient launch
-  const bool wall_in_grad = p->nd.prec == WMX_PREC_FP32_BULK && !p->nd.net.w;
+  }
   WMX_CHECK(timed(p, K_GRAD, s, [&] {
     return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd,
-                       wall_in_grad ? p->d_wall : nullptr, (int)p->wall.size(), p->wall_max, q, s);
+                       wall_in_grad ? p->d_wall : nullptr, (int)p->wall.size(), p->wall_max, q, gdt, s);
   }));
   if (!wall_in_grad)
     WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));

@@ -118,9 +118,19 @@ hipError_t launch_geometry(const double* X, const int* N, double* vol, double* x
 // launch is skipped) instead of read from q
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
                        const NumDev& nd, double* d_part, int zfuse, hipStream_t s);
-// FP32-bulk plans: d_wall != nullptr also evaluates those Newton wall faces in the launch
+// per-case dt mins taken inside the gradient launch (out == nullptr: none)
+struct GradDt {
+  int ncases = 0;
+  const int* list = nullptr;       // per-case workgroup list (nullptr: identity)
+  const int* off = nullptr;        // per-case offsets into the list
+  const double* part = nullptr;    // CFL partials
+  double* out = nullptr;           // dt per case
+};
+// FP32-bulk plans: d_wall != nullptr also evaluates those Newton wall faces in the launch,
+// gdt.out != nullptr the per-case dt mins
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
-                       const WallOp* d_wall, int nwall_ops, int wall_max, Ptrs q, hipStream_t s);
+                       const WallOp* d_wall, int nwall_ops, int wall_max, Ptrs q, const GradDt& gdt,
+                       hipStream_t s);
 hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, int maxcount, Ptrs q,
                        const NumDev& nd, hipStream_t s);
 hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles, int ci_len,

@@ -180,23 +180,67 @@ __device__ __forceinline__ void wall_face(const BlockDev* __restrict__ blks, con
 // that every plane stays below 4 GiB), so loads take the scalar-base form and need one
 // 32-bit offset register instead of a 64-bit address pair each.  The SGS model is a
 // template parameter so that only one model is compiled into each instance.
-// wall_ops != nullptr: the first nwall workgroups (a multiple of 8, so the gradient
-// workgroups keep their XCD spans) evaluate the Newton wall faces, wall_wgx workgroups per
-// wall op, alongside the memory-bound gradient workgroups.
+// NaN-propagating min (numpy.min semantics)
+__device__ __forceinline__ double nan_min(double a, double b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  return a < b ? a : b;
+}
+
+// dt of case cs: the min of the case's CFL partials (list == nullptr: the identity list),
+// one 256-thread workgroup, eight independent loads in flight per thread
+__device__ void case_dt_min(const int* __restrict__ list, const int* __restrict__ off,
+                            const double* __restrict__ part, int cs, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int lo = off[cs], hi = off[cs + 1];
+  double a = __builtin_huge_val();
+  for (int t0 = lo + (int)threadIdx.x; t0 < hi; t0 += 8 * (int)blockDim.x) {
+    double va[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u * (int)blockDim.x;
+      va[u] = t < hi ? part[list ? list[t] : t] : __builtin_huge_val();
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a = nan_min(a, va[u]);
+  }
+  for (int o = 32; o > 0; o >>= 1) a = nan_min(a, __shfl_xor(a, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = nan_min(r, red[w]);
+    out[cs] = r;
+  }
+}
+
+// The launch's first `head` workgroups (a multiple of 8, so the gradient workgroups keep
+// their XCD spans) do latency-bound side work beside the memory-bound gradients:
+//   [0, ndt): the per-case dt min of the CFL partials (the step's first stage);
+//   [ndt, ndt + wall_wgx * nwall_ops): the Newton wall faces, wall_wgx per wall op.
 template <int SGS>
 __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blks,
                                                 const int* __restrict__ wg_prefix, int nblk,
-                                                NumDev nd, const WallOp* __restrict__ wall_ops,
-                                                int nwall, int wall_wgx, int nwall_ops, Ptrs q) {
-  if ((int)blockIdx.x < nwall) {
-    const int op_i = (int)blockIdx.x / wall_wgx;
-    if (op_i >= nwall_ops) return;                        // padding workgroups
+                                                NumDev nd, int head, const WallOp* __restrict__ wall_ops,
+                                                int wall_wgx, int nwall_ops, Ptrs q, int ndt,
+                                                const int* __restrict__ dt_list,
+                                                const int* __restrict__ dt_off,
+                                                const double* __restrict__ dt_part,
+                                                double* __restrict__ dt_out) {
+  if ((int)blockIdx.x < head) {
+    const int h = (int)blockIdx.x;
+    if (h < ndt) {
+      case_dt_min(dt_list, dt_off, dt_part, h, dt_out);
+      return;
+    }
+    const int op_i = (h - ndt) / max(wall_wgx, 1);
+    if (wall_wgx <= 0 || op_i >= nwall_ops) return;      // padding workgroups
     const WallOp op = wall_ops[op_i];
-    const int idx0 = ((int)blockIdx.x - op_i * wall_wgx) * (int)blockDim.x + (int)threadIdx.x;
+    const int idx0 = (h - ndt - op_i * wall_wgx) * (int)blockDim.x + (int)threadIdx.x;
     if (idx0 < op.count) wall_face<false>(blks, op, idx0, q, nd, nullptr, nullptr);
     return;
   }
-  const int wg = xcd_span((int)blockIdx.x - nwall, (int)gridDim.x - nwall);
+  const int wg = xcd_span((int)blockIdx.x - head, (int)gridDim.x - head);
   const int b = find_block(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
   const int ng = B.ng;
@@ -326,14 +370,18 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
     f2 += f2s0;
   }
 }
-template __global__ void k_grad32<WMX_SGS_NONE>(const BlockDev*, const int*, int, NumDev,
-                                                  const WallOp*, int, int, int, Ptrs);
-template __global__ void k_grad32<WMX_SGS_SMAGORINSKY>(const BlockDev*, const int*, int, NumDev,
-                                                  const WallOp*, int, int, int, Ptrs);
-template __global__ void k_grad32<WMX_SGS_WALE>(const BlockDev*, const int*, int, NumDev,
-                                                  const WallOp*, int, int, int, Ptrs);
-template __global__ void k_grad32<WMX_SGS_VREMAN>(const BlockDev*, const int*, int, NumDev,
-                                                  const WallOp*, int, int, int, Ptrs);
+template __global__ void k_grad32<WMX_SGS_NONE>(const BlockDev*, const int*, int, NumDev, int,
+                                                  const WallOp*, int, int, Ptrs, int, const int*,
+                                                  const int*, const double*, double*);
+template __global__ void k_grad32<WMX_SGS_SMAGORINSKY>(const BlockDev*, const int*, int, NumDev, int,
+                                                  const WallOp*, int, int, Ptrs, int, const int*,
+                                                  const int*, const double*, double*);
+template __global__ void k_grad32<WMX_SGS_WALE>(const BlockDev*, const int*, int, NumDev, int,
+                                                  const WallOp*, int, int, Ptrs, int, const int*,
+                                                  const int*, const double*, double*);
+template __global__ void k_grad32<WMX_SGS_VREMAN>(const BlockDev*, const int*, int, NumDev, int,
+                                                  const WallOp*, int, int, Ptrs, int, const int*,
+                                                  const int*, const double*, double*);
 
 // ------------------------------------------------------------------------------ wall
 // NET: the learned model (wave-cooperative network + Newton polish) replaces the
@@ -444,30 +492,25 @@ hipError_t launch_wm_eval(const NumDev& nd, int mode, long long n, const double*
 }
 
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
-                       const WallOp* d_wall, int nwall_ops, int wall_max, Ptrs q, hipStream_t s) {
+                       const WallOp* d_wall, int nwall_ops, int wall_max, Ptrs q, const GradDt& gdt,
+                       hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   if (nd.prec == WMX_PREC_FP32_BULK) {
-    // Newton wall faces in the same launch (workgroups first, padded to a multiple of 8)
+    // side work in the same launch: per-case dt mins, then Newton wall faces (padded to 8)
     const int wx = d_wall && nwall_ops > 0 && wall_max > 0 ? (wall_max + 255) / 256 : 0;
-    const int nwall = ((wx * nwall_ops + 7) / 8) * 8;
-    const dim3 grid(nwg + nwall);
+    const int ndt = gdt.out ? gdt.ncases : 0;
+    const int head = ((ndt + wx * nwall_ops + 7) / 8) * 8;
+    const dim3 grid(nwg + head);
+#define WMX_GRAD32(S)                                                                          \
+  hipLaunchKernelGGL(k_grad32<S>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, nd, head, d_wall, \
+                     wx, nwall_ops, q, ndt, gdt.list, gdt.off, gdt.part, gdt.out)
     switch (nd.sgs) {
-      case WMX_SGS_NONE:
-        hipLaunchKernelGGL(k_grad32<WMX_SGS_NONE>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, nd,
-                           d_wall, nwall, wx, nwall_ops, q);
-        break;
-      case WMX_SGS_SMAGORINSKY:
-        hipLaunchKernelGGL(k_grad32<WMX_SGS_SMAGORINSKY>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, nd,
-                           d_wall, nwall, wx, nwall_ops, q);
-        break;
-      case WMX_SGS_WALE:
-        hipLaunchKernelGGL(k_grad32<WMX_SGS_WALE>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, nd,
-                           d_wall, nwall, wx, nwall_ops, q);
-        break;
-      default:
-        hipLaunchKernelGGL(k_grad32<WMX_SGS_VREMAN>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, nd,
-                           d_wall, nwall, wx, nwall_ops, q);
+      case WMX_SGS_NONE: WMX_GRAD32(WMX_SGS_NONE); break;
+      case WMX_SGS_SMAGORINSKY: WMX_GRAD32(WMX_SGS_SMAGORINSKY); break;
+      case WMX_SGS_WALE: WMX_GRAD32(WMX_SGS_WALE); break;
+      default: WMX_GRAD32(WMX_SGS_VREMAN);
     }
+#undef WMX_GRAD32
   } else
     hipLaunchKernelGGL(k_grad_cell<double>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
   return hipGetLastError();
