@@ -75,6 +75,10 @@ struct wmx_plan {
   std::vector<std::vector<int>> phase_inflow;  // block ids of inflow fills per phase
   bool fuse_last = false;   // the last phase is k-face fills only: a step's primitive launches
                             // evaluate those ghosts (BlockDev::zfuse) and skip its fill launch
+  // sponge row means in head workgroups of the primitive launch (every block's rows of
+  // 9..129 cells): per-block first workgroup, 32 rows each
+  std::vector<int> sponge_pre;
+  int* d_sponge_pre = nullptr;
   // wall faces
   std::vector<WallOp> wall;
   WallOp* d_wall = nullptr;
@@ -434,9 +438,14 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
               double* dt_out = nullptr, bool zfuse = false) {
   zfuse = zfuse && p->fuse_last;
   if (run_fill(p, q, time, d_dt, coef, s, zfuse)) return 1;
+  SpongeHead sh;
+  if (p->d_sponge_pre) {
+    sh.pre = p->d_sponge_pre;
+    sh.nwg = p->sponge_pre.back();
+  }
   WMX_CHECK(timed(p, K_PRIM, s, [&] {
     return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd,
-                       dt_out ? p->d_part : nullptr, zfuse ? 1 : 0, s);
+                       dt_out ? p->d_part : nullptr, zfuse ? 1 : 0, sh, s);
   }));
   // FP32-bulk plans take the per-case dt mins and (Newton model) the wall faces into the
   // gradient launch; the flux kernel after it is the first reader of both
@@ -461,7 +470,7 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
   }));
   if (!wall_in_grad)
     WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
-  if (p->nd.n_sponge > 0)
+  if (p->nd.n_sponge > 0 && !p->d_sponge_pre)
     for (int b = 0; b < p->nb; ++b)
       WMX_CHECK(timed(p, K_SPONGE, s, [&] {
         return launch_sponge_mean(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], q, s);
@@ -674,6 +683,16 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   };
   p->case_wg_ident = ident(p->case_wg);
   p->case_wg_all_ident = ident(p->case_wg_all);
+  // sponge means in the primitive launch when every block's k rows suit the 8-lane form
+  if (nd.n_sponge > 0) {
+    bool ok = true;
+    for (const BlockDev& B : p->hb) ok = ok && B.n[2] - 1 >= 8 && B.n[2] - 1 <= 128;
+    if (ok) {
+      p->sponge_pre.assign(1, 0);
+      for (const BlockDev& B : p->hb)
+        p->sponge_pre.push_back(p->sponge_pre.back() + (int)(((long long)B.n[0] * B.n[1] + 31) / 32));
+    }
+  }
   // workspace layout
   size_t off = 0;
   p->off_wall.assign(nblocks, std::vector<long long>(6, -1));
@@ -721,6 +740,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
       upload(p->pre_int, &p->d_pre_int) || upload(p->pre_shell, &p->d_pre_shell) ||
       upload(p->case_wg, &p->d_case_wg) || upload(p->case_off, &p->d_case_off) ||
       upload(p->case_wg_all, &p->d_case_wg_all) || upload(p->case_off_all, &p->d_case_off_all) ||
+      (!p->sponge_pre.empty() && upload(p->sponge_pre, &p->d_sponge_pre)) ||
       upload(cdir, &p->d_case_dir) || upload(p->case_target, &p->d_case_target)) {
     std::string m = g_err;
     wmx_plan_destroy(p);
@@ -750,6 +770,7 @@ int wmx_plan_destroy(wmx_plan* p) {
   (void)hipFree(p->d_case_off);
   (void)hipFree(p->d_case_wg_all);
   (void)hipFree(p->d_case_off_all);
+  (void)hipFree(p->d_sponge_pre);
   (void)hipFree(p->d_case_dir);
   (void)hipFree(p->d_case_target);
   delete p;

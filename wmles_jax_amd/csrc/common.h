@@ -114,10 +114,17 @@ hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);
 // d_part != nullptr: the launch also writes the CFL partial of each workgroup (step fusion)
+// sponge row means taken by head workgroups of a primitive launch (pre == nullptr: none):
+// pre[b] = first head workgroup of block b (32 rows each), nwg = their total
+struct SpongeHead {
+  const int* pre = nullptr;
+  int nwg = 0;
+  int head = 0;   // set by the launcher
+};
 // zfuse: the k ghosts flagged in BlockDev::zfuse are evaluated from the sweep (their fill
 // launch is skipped) instead of read from q
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
-                       const NumDev& nd, double* d_part, int zfuse, hipStream_t s);
+                       const NumDev& nd, double* d_part, int zfuse, SpongeHead sh, hipStream_t s);
 // per-case dt mins taken inside the gradient launch (out == nullptr: none)
 struct GradDt {
   int ncases = 0;

@@ -120,6 +120,65 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
   if (threadIdx.x == 0) part[wg] = r;
 }
 
+// Spanwise mean of sponge row `row` for rows of 9..129 cells (numpy's pairwise order for
+// the elements 1..nk-1 by eight lanes u of the row, combined by shuffles in that order);
+// every lane of the 8-lane group calls it.  smooth_ramp.py:214-216.
+__device__ __forceinline__ void sponge_row_fast(const BlockDev& B, long long row, int u, bool live,
+                                                long long c0, const double* __restrict__ qb) {
+  const long long rows = (long long)B.n[0] * B.n[1];
+  const int n = B.n[2] - 1;
+  const double* vol = B.vol + c0;
+  const long long cells = B.cells;
+  double a[6];
+  const int nfull = n - n % 8;
+  if (live) {
+    const double vv = vol[1 + u];
+    a[0] = vv;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a[1 + v] = qb[v * cells + c0 + 1 + u] * vv;
+    for (int i = 8; i < nfull; i += 8) {
+      const double w = vol[1 + i + u];
+      a[0] = a[0] + w;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i + u] * w;
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 6; ++v) a[v] = 0.0;
+  }
+#pragma unroll
+  for (int m = 1; m <= 4; m <<= 1)
+#pragma unroll
+    for (int v = 0; v < 6; ++v) a[v] = a[v] + __shfl_xor(a[v], m, 8);
+  if (!live || u != 0) return;
+  for (int i = nfull; i < n; ++i) {
+    const double w = vol[1 + i];
+    a[0] = a[0] + w;
+    for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i] * w;
+  }
+  const double den = vol[0] + a[0];
+  for (int v = 0; v < 5; ++v) B.sponge_mean[v * rows + row] = (qb[v * cells + c0] * vol[0] + a[1 + v]) / den;
+}
+
+// Head workgroups of a primitive launch (a multiple of 8, so the primitive workgroups keep
+// their XCD spans): workgroup h < sh.nwg takes 32 sponge rows of the block whose range
+// sh.pre[b] <= h < sh.pre[b + 1] holds it (rows of 9..129 cells only).
+__device__ __forceinline__ void sponge_head(const BlockDev* __restrict__ blks, int nblk, Ptrs q,
+                                            const SpongeHead& sh) {
+  const int h = (int)blockIdx.x;
+  if (h >= sh.nwg) return;
+  int b = 0;
+  while (b + 1 < nblk && sh.pre[b + 1] <= h) ++b;
+  const BlockDev& B = blks[b];
+  const long long rows = (long long)B.n[0] * B.n[1];
+  const long long row = (long long)(h - sh.pre[b]) * 32 + (threadIdx.x >> 3);
+  const int u = threadIdx.x & 7;
+  const bool live = row < rows && B.sponge_row[row];
+  const int ng = B.ng;
+  const long long c0 = live ? cidx(B.N, ng + (int)(row / B.n[1]), ng + (int)(row % B.n[1]), ng) : 0;
+  sponge_row_fast(B, row, u, live, c0, q.p[b]);
+}
+
 // ------------------------------------------------------------------------------ prim
 // Primitive records of every cell (ghosts included).  DT: the stage-1 launch of a step
 // also takes the CFL partial of its workgroup's interior cells into part[workgroup],
@@ -164,10 +223,14 @@ template <bool DT>
 __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
                                               const int* __restrict__ wg_prefix, int nblk,
                                               Ptrs q, NumDev nd, double* __restrict__ part,
-                                              int zfuse) {
+                                              int zfuse, SpongeHead sh) {
+  if ((int)blockIdx.x < sh.head) {
+    sponge_head(blks, nblk, q, sh);
+    return;
+  }
   // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
   // stay in one L2
-  const int wg = xcd_span(blockIdx.x, gridDim.x);
+  const int wg = xcd_span((int)blockIdx.x - sh.head, (int)gridDim.x - sh.head);
   const int b = find_block_r(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
   const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
@@ -202,10 +265,14 @@ template <bool DT>
 __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blks,
                                                 const int* __restrict__ wg_prefix, int nblk,
                                                 Ptrs q, NumDev nd, double* __restrict__ part,
-                                                int zfuse) {
+                                                int zfuse, SpongeHead sh) {
+  if ((int)blockIdx.x < sh.head) {
+    sponge_head(blks, nblk, q, sh);
+    return;
+  }
   // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
   // stay in one L2
-  const int wg = xcd_span(blockIdx.x, gridDim.x);
+  const int wg = xcd_span((int)blockIdx.x - sh.head, (int)gridDim.x - sh.head);
   const int b = find_block_r(wg_prefix, nblk, wg);
   const BlockDev& B = blks[b];
   const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
@@ -423,35 +490,7 @@ __global__ __launch_bounds__(256) void k_sponge_mean(const BlockDev* __restrict_
   const long long cells = B.cells;
   const int n = nk - 1;                                  // the pairwise part: elements 1..nk-1
   if (n <= 128 && n >= 8) {
-    double a[6];
-    const int nfull = n - n % 8;
-    if (live) {
-      const double vv = vol[1 + u];
-      a[0] = vv;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) a[1 + v] = qb[v * cells + c0 + 1 + u] * vv;
-      for (int i = 8; i < nfull; i += 8) {
-        const double w = vol[1 + i + u];
-        a[0] = a[0] + w;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i + u] * w;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 6; ++v) a[v] = 0.0;
-    }
-#pragma unroll
-    for (int m = 1; m <= 4; m <<= 1)
-#pragma unroll
-      for (int v = 0; v < 6; ++v) a[v] = a[v] + __shfl_xor(a[v], m, 8);
-    if (!live || u != 0) return;
-    for (int i = nfull; i < n; ++i) {
-      const double w = vol[1 + i];
-      a[0] = a[0] + w;
-      for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i] * w;
-    }
-    const double den = vol[0] + a[0];
-    for (int v = 0; v < 5; ++v) B.sponge_mean[v * rows + row] = (qb[v * cells + c0] * vol[0] + a[1 + v]) / den;
+    sponge_row_fast(B, row, u, live, c0, qb);
     return;
   }
   if (!live || u != 0) return;                           // short or long rows: one lane per row
@@ -472,15 +511,18 @@ hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk,
 }
 
 hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
-                       const NumDev& nd, double* d_part, int zfuse, hipStream_t s) {
+                       const NumDev& nd, double* d_part, int zfuse, SpongeHead sh, hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   const bool f32 = nd.prec == WMX_PREC_FP32_BULK;
+  sh.head = sh.pre ? ((sh.nwg + 7) / 8) * 8 : 0;
+  if (!sh.pre) sh.nwg = 0;
+  const dim3 grid(nwg + sh.head);
   if (d_part) {
-    if (f32) hipLaunchKernelGGL(k_prim32<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
-    else hipLaunchKernelGGL(k_prim<true>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
+    if (f32) hipLaunchKernelGGL(k_prim32<true>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse, sh);
+    else hipLaunchKernelGGL(k_prim<true>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse, sh);
   } else {
-    if (f32) hipLaunchKernelGGL(k_prim32<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
-    else hipLaunchKernelGGL(k_prim<false>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse);
+    if (f32) hipLaunchKernelGGL(k_prim32<false>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse, sh);
+    else hipLaunchKernelGGL(k_prim<false>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse, sh);
   }
   return hipGetLastError();
 }
