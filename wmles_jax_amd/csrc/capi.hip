@@ -626,6 +626,10 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
       for (int o = p->phase_off[nph - 1]; o < p->phase_off[nph]; ++o) {
         const GhostOp& op = p->ops[o];
         if (op.axis != 2 || p->hb[op.blk].faces[4 + op.side].kind == WMX_BC_INFLOW) ok = false;
+        // no earlier phase writes those k ghosts either (then a step's first stage leaves
+        // them unwritten in its input, which the ghost-shell kernels rely on)
+        for (int e = 0; ok && e < p->phase_off[nph - 1]; ++e)
+          if (p->ops[e].axis == 2 && p->ops[e].blk == op.blk && p->ops[e].side == op.side) ok = false;
       }
     p->fuse_last = ok;
     if (ok)

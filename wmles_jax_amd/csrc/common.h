@@ -296,6 +296,15 @@ WMX_HD void shell_cell(long long s, const int* N, const int* n, int ng, int* I, 
   *K = l < ng ? (int)l : (int)(n[2] + l);
 }
 
+// k ghost of shell cell (I, J, K) whose side the step's primitive launches evaluate (the
+// sweep's last phase, BlockDev::zfuse): a step's first stage leaves it unwritten in q0
+WMX_HD bool shell_kept(const int* n, int ng, int zfuse, int K) {
+  if (!zfuse) return false;
+  if (K < ng) return zfuse & 1;
+  if (K >= ng + n[2]) return (zfuse >> 1) & 1;
+  return false;
+}
+
 WMX_HD long long shell_count(const int* N, const int* n, int ng) {
   return 2LL * ng * ((long long)N[1] * N[2] + (long long)n[0] * N[2] + (long long)n[0] * n[1]);
 }

@@ -413,6 +413,10 @@ __global__ void k_shell_save(const BlockDev* __restrict__ blks, const int* __res
   if (s >= B.shell_cells) return;
   int I, J, K;
   shell_cell(s, B.N, B.n, B.ng, &I, &J, &K);
+  // (interior-(I, J) k ghosts of fused sides: q0 keeps them through the step)
+  if (I >= B.ng && I < B.ng + B.n[0] && J >= B.ng && J < B.ng + B.n[1] &&
+      shell_kept(B.n, B.ng, B.zfuse, K))
+    return;
   const long long c = cidx(B.N, I, J, K);
   for (int v = 0; v < 5; ++v) B.shell[v * B.shell_cells + s] = q.p[b][v * B.cells + c];
 }
@@ -436,9 +440,12 @@ __global__ void k_shell_finish(const BlockDev* __restrict__ blks, const int* __r
   }
   const double dt = d_dt ? d_dt[B.case_id] : dt_host;
   const double zr = dt * 0.0;
+  // interior-(I, J) k ghosts of fused sides were never written in q0: read there, no restore
+  const bool kept = I >= B.ng && I < B.ng + B.n[0] && J >= B.ng && J < B.ng + B.n[1] &&
+                    shell_kept(B.n, B.ng, B.zfuse, K);
   for (int v = 0; v < 5; ++v) {
-    const double g = B.shell[v * B.shell_cells + s];
-    q0.p[b][v * B.cells + c] = g;
+    const double g = kept ? q0.p[b][v * B.cells + c] : B.shell[v * B.shell_cells + s];
+    if (!kept) q0.p[b][v * B.cells + c] = g;
     const double q1 = g + zr;
     const double q2 = 0.75 * g + 0.25 * (q1 + zr);
     out.p[b][v * B.cells + c] = (1.0 / 3.0) * g + (2.0 / 3.0) * (q2 + zr);
