@@ -189,10 +189,21 @@ __global__ void k_inflow(const BlockDev* __restrict__ blks, int blk, InflowDev i
     double f = 0.0;
     if (coop) {
       __syncthreads();                                   // yz staged / previous sums done
-      for (int e = threadIdx.x; e < N2 * nm; e += blockDim.x) {
-        const int kk = e / nm, k = e - kk * nm;
-        const double ph = ((md[k] * X0 + md[nm + k] * yz[kk]) + md[2 * nm + k] * yz[N2 + kk]) + md[3 * nm + k];
-        cosb[e] = cos(ph);
+      if ((int)blockDim.x % nm == 0) {
+        // every element of this thread has the same mode k: its coefficients (and the
+        // time term) are loaded once; the phase keeps its operation order
+        const int k = (int)threadIdx.x % nm;
+        const double ax = md[k] * X0, by = md[nm + k], cz = md[2 * nm + k], dk = md[3 * nm + k];
+        for (int kk = (int)threadIdx.x / nm; kk < N2; kk += (int)blockDim.x / nm) {
+          const double ph = ((ax + by * yz[kk]) + cz * yz[N2 + kk]) + dk;
+          cosb[kk * nm + k] = cos(ph);
+        }
+      } else {
+        for (int e = threadIdx.x; e < N2 * nm; e += blockDim.x) {
+          const int kk = e / nm, k = e - kk * nm;
+          const double ph = ((md[k] * X0 + md[nm + k] * yz[kk]) + md[2 * nm + k] * yz[N2 + kk]) + md[3 * nm + k];
+          cosb[e] = cos(ph);
+        }
       }
       __syncthreads();
       if (valid) {
