@@ -610,6 +610,10 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     }
   for (int i = 0; i < ninflow; ++i) {
     const wmx_inflow_desc& in = inflows[i];
+    if (in.n_tab < 1 || in.n_modes < 1 || !in.table || !in.modes) {
+      delete p;
+      return fail("inflow needs a profile table and at least one mode");
+    }
     p->inflows.push_back(InflowDev{in.n_tab, in.n_modes, 0, in.table, in.modes, in.U_inf, in.L,
                                    in.rho_inf, in.p_inf});
   }

@@ -133,17 +133,29 @@ __global__ void k_geom_cells(const double* __restrict__ X, int N0, int N1, int N
   }
 }
 
-// numpy.interp semantics for an increasing table column (left/right clamp).
-__device__ double interp_np(double x, const double* xp, const double* fp, int stride, int n) {
-  if (x < xp[0]) return fp[0];
-  if (x > xp[(n - 1) * stride]) return fp[(n - 1) * stride];
-  if (x == xp[(n - 1) * stride]) return fp[(n - 1) * stride];
+// numpy.interp semantics for an increasing table column (left/right clamp), split in two:
+// where x falls in the abscissa column (the same for every column of a table, so one
+// search serves them all), then the value of one column there.
+struct InterpAt {
+  int lo;
+  bool exact;                                  // fp[lo] itself (clamped, or x on a node)
+};
+
+__device__ InterpAt interp_at(double x, const double* xp, int stride, int n) {
+  if (x < xp[0]) return {0, true};
+  if (x > xp[(n - 1) * stride]) return {n - 1, true};
+  if (x == xp[(n - 1) * stride]) return {n - 1, true};
   int lo = 0, hi = n - 1;                      // xp[lo] <= x < xp[hi]
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (xp[mid * stride] <= x) lo = mid; else hi = mid;
   }
-  if (xp[lo * stride] == x) return fp[lo * stride];
+  return {lo, xp[lo * stride] == x || n < 2};   // n < 2: only a NaN x gets here
+}
+
+__device__ double interp_col(InterpAt at, double x, const double* xp, const double* fp, int stride) {
+  const int lo = at.lo;
+  if (at.exact) return fp[lo * stride];
   double slope = (fp[(lo + 1) * stride] - fp[lo * stride]) / (xp[(lo + 1) * stride] - xp[lo * stride]);
   return slope * (x - xp[lo * stride]) + fp[lo * stride];
 }
@@ -169,12 +181,19 @@ __global__ void k_inflow(const BlockDev* __restrict__ blks, int blk, InflowDev i
   double z_l = B.xc[2 * cells + ci] / in.L;
   double X0 = in.U_inf * t / in.L;
   const bool inner = valid && K >= ng && K < ng + nk;
+  // the profile columns at y_l, loaded before the mode sums (one search of the y column)
+  const double* tb = in.table;
+  const InterpAt at = interp_at(y_l, tb, 7, in.n_tab);
+  double tcol[6];
+  for (int c = 0; c < 6; ++c) tcol[c] = interp_col(at, y_l, tb, tb + 1 + c, 7);
   const int nm = in.n_modes;
   // cos of every (cell, mode) phase, computed by all threads of the workgroup into LDS
   // (when the row fits), then summed per cell in mode order: the same values and the same
-  // sum as one thread per cell doing its modes in turn
-  double* cosb = red + blockDim.x + 1;                  // [N2][nm] when cooperative
-  double* yz = cosb + (long long)N2 * nm;               // [2][N2]
+  // sum as one thread per cell doing its modes in turn.  Mode-major with an odd pitch, so
+  // that lanes of one mode (the sums) or of one cell (the phases) fall in distinct banks.
+  const int P2 = N2 | 1;
+  double* cosb = red + blockDim.x + 1;                  // [nm][P2] when cooperative
+  double* yz = cosb + (long long)P2 * nm;               // [2][N2]
   const bool coop = in.coop;
   if (coop) {
     for (int kk = threadIdx.x; kk < N2; kk += blockDim.x) {
@@ -196,20 +215,19 @@ __global__ void k_inflow(const BlockDev* __restrict__ blks, int blk, InflowDev i
         const double ax = md[k] * X0, by = md[nm + k], cz = md[2 * nm + k], dk = md[3 * nm + k];
         for (int kk = (int)threadIdx.x / nm; kk < N2; kk += (int)blockDim.x / nm) {
           const double ph = ((ax + by * yz[kk]) + cz * yz[N2 + kk]) + dk;
-          cosb[kk * nm + k] = cos(ph);
+          cosb[k * P2 + kk] = cos(ph);
         }
       } else {
         for (int e = threadIdx.x; e < N2 * nm; e += blockDim.x) {
           const int kk = e / nm, k = e - kk * nm;
           const double ph = ((md[k] * X0 + md[nm + k] * yz[kk]) + md[2 * nm + k] * yz[N2 + kk]) + md[3 * nm + k];
-          cosb[e] = cos(ph);
+          cosb[k * P2 + kk] = cos(ph);
         }
       }
       __syncthreads();
       if (valid) {
-        const double* cr = cosb + (long long)K * nm;
-        double acc = cr[0];
-        for (int k = 1; k < nm; ++k) acc = acc + cr[k];
+        double acc = cosb[K];
+        for (int k = 1; k < nm; ++k) acc = acc + cosb[k * P2 + K];
         f = acc;
       }
     } else if (valid) {
@@ -248,9 +266,7 @@ __global__ void k_inflow(const BlockDev* __restrict__ blks, int blk, InflowDev i
   double var2 = row_mean(fld[2] * fld[2]);
   double n2 = fld[2] / sqrt(fmax(var2, 1e-20));
   if (!valid) return;
-  const double* tb = in.table;
-  const int nt = in.n_tab;
-  auto col = [&](int c) { return interp_np(y_l, tb, tb + c, 7, nt); };
+  auto col = [&](int c) { return tcol[c - 1]; };
   double Ur = in.U_inf, U2 = Ur * Ur;
   double uu = fmax(col(3), 0.0) * U2, vv = fmax(col(4), 0.0) * U2, ww = fmax(col(5), 0.0) * U2;
   double uv = col(6) * U2;
@@ -290,7 +306,7 @@ hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng
   if (threads > 1024) return hipErrorInvalidValue;
   dim3 grid(N1, ng);
   // cooperative phase evaluation when a row's (cell, mode) table fits in 56 KB of LDS
-  const size_t coop_bytes = sizeof(double) * ((size_t)N2 * in.n_modes + 2 * (size_t)N2);
+  const size_t coop_bytes = sizeof(double) * ((size_t)(N2 | 1) * in.n_modes + 2 * (size_t)N2);
   in.coop = coop_bytes <= 56 * 1024 ? 1 : 0;
   if (in.coop && threads < 256) threads = 256;           // more lanes for the phase table
   size_t shm = sizeof(double) * (threads + 1) + (in.coop ? coop_bytes : 0);
