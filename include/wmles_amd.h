@@ -110,9 +110,9 @@ typedef struct wmx_numerics_desc {
 
 typedef struct wmx_inflow_desc {
   /* reference smooth_ramp.py:116-197 (SmoothRampInflow) */
-  int32_t n_tab;           /* rows of the profile table                         */
+  int32_t n_tab;           /* rows of the profile table (>= 1)                  */
   const double* table;     /* device (n_tab, 7): y/L, U, V, uu, vv, ww, uv      */
-  int32_t n_modes;
+  int32_t n_modes;         /* >= 1; wmx_plan_create rejects 0 or a NULL array   */
   const double* modes;     /* device (3 components, 4, n_modes): kx, ky, kz, phase */
   double U_inf, L, rho_inf, p_inf;
 } wmx_inflow_desc;

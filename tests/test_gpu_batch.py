@@ -178,3 +178,22 @@ def test_step_with_inflow_equals_max_dt_plus_advance(precision):
         assert float(dta) == float(dtb)
         for a, b in zip(A, B):
             assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("field", ["n_tab", "n_modes"])
+def test_plan_rejects_an_empty_inflow_table(field):
+    """An inflow descriptor with no profile rows or no modes is refused by wmx_plan_create
+    with a message (the fill's table search needs a row, its phase sum a mode)."""
+    import ctypes as C
+    from wmles_jax_amd import _lib
+    spec, arrays, prim = cases.ramp_spec(wmx, 48, 24, 16, n_modes=8)
+    mesh, s = cases.build(wmx, spec, arrays)
+    inf = s._ctx.inflows[0]
+    descs = (_lib.InflowDesc * 1)()
+    descs[0] = inf.desc(s._dev)
+    setattr(descs[0], field, 0)
+    plan = C.c_void_p()
+    with pytest.raises(_lib.WmxError, match="profile table and at least one mode"):
+        _lib.check(_lib.lib.wmx_plan_create(len(s._descs), s._descs, C.byref(s._nd), 1, descs,
+                                            C.byref(plan)))
+    assert not plan.value
