@@ -367,3 +367,30 @@ def test_redundant_sweep_passes_dropped_only_when_exact():
                         golden_io.blocks_of(ent, "Qbc0")):
             np.testing.assert_array_equal(q, g)
     print("sweep passes kept:", kept)
+
+
+def test_inflow_profile_lookup_exact_nodes_and_clamps():
+    """The inflow's profile lookup (numpy.interp semantics: one bracket search of the y
+    column, shared by the six value columns) against the oracle, with a table whose y
+    column holds ghost-cell centroids exactly (462 of the 1980 ghost cells sit on a node)
+    and spans less than the ghosted rows (cells below the first and above the last
+    node); ghost values at the golden ramp's 1e-13 bar."""
+    spec, arrays, prim = cases.ramp_spec(wmx, 48, 24, 16, n_modes=8)
+    L, ng = spec["ramp_props"]["L"], spec["ng"]
+    m0, _ = cases.build(oracle, spec, arrays)
+    yl = np.asarray(m0.geoms[0].xc)[1, :ng] / L          # the inflow ghost cells' y / L
+    yt = np.unique(np.concatenate([yl[:, ng + 2:-ng - 2:3, :].ravel(), np.linspace(0.25, 0.7, 9)]))
+    assert np.isin(yl, yt).sum() > 0 and yl.min() < yt[0] and yl.max() > yt[-1]
+    a = 0.08 ** 2
+    prof = np.stack([yt, 0.5 + 0.5 * np.tanh((yt - 0.3) / 0.05), 0.01 * np.sin(9.0 * yt),
+                     a * (1.0 + 0.3 * np.sin(20.0 * yt)), a * (1.0 + 0.2 * np.cos(15.0 * yt)),
+                     0.8 * a + 0.0 * yt, -0.3 * a * (1.0 + 0.1 * yt)], axis=1)
+    arrays = dict(arrays, inflow_profile=prof)
+    _, s0 = cases.build(oracle, spec, arrays)
+    _, s1 = cases.build(wmx, spec, arrays)
+    Q = s1.init_state(prim)
+    t = 0.05 * L / spec["ramp_props"]["U_inf"]
+    A = host(s1.apply_bcs(Q, s1.geoms, t))
+    B = s0.apply_bcs(host(Q), s0.geoms, t)
+    for qa, qb in zip(A, B):
+        assert golden_io.rel_err(qa[:, :ng], np.asarray(qb)[:, :ng]) <= 1e-13
