@@ -5,7 +5,8 @@
 Every translation unit is compiled with packed FP32 disabled.  The ghost-fill /
 geometry and reduction units are compiled without floating-point contraction so
 that their results round exactly like the reference's float64 host arithmetic; the
-RHS unit contracts multiply-adds into FMAs.
+RHS unit contracts multiply-adds into FMAs.  The FP64 and Hybrid FP32-bulk flux kernels
+(flux_ilp.hip) are scheduled by LLVM's max-ILP machine scheduler.
 """
 import os
 import subprocess
@@ -29,6 +30,7 @@ UNITS = {
     "reduce.hip": ["-ffp-contract=off"],
     "rhs.hip": ["-ffp-contract=fast"],
     "flux.hip": ["-ffp-contract=fast"],
+    "flux_ilp.hip": ["-ffp-contract=fast", "-mllvm", "-amdgpu-sched-strategy=max-ilp"],
     "mlp.hip": ["-ffp-contract=fast"],
     "capi.hip": [],
 }

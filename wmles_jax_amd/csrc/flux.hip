@@ -782,14 +782,25 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
   }
 }
 
-template __global__ void k_flux1<0>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
-                                   const double*, double);
-template __global__ void k_flux1<1>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
-                                   const double*, double);
-template __global__ void k_flux1<2>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
-                                   const double*, double);
-template __global__ void k_flux1<3>(const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs,
-                                   const double*, double);
+// The FP64 kernels and the Hybrid (KEP4 + MUSCL-HLLC) FP32-bulk kernels are compiled in
+// flux_ilp.hip (this file with WMX_FLUX_ILP_UNIT defined) under LLVM's max-ILP machine
+// scheduler, which shortens them by 2-3% (the arithmetic is the same); the other FP32-bulk
+// kernels, which it slows, are instantiated here by their launches.
+#define WMX_FLUX_SIG (const BlockDev*, const int4*, int, NumDev, Ptrs, Ptrs, Ptrs, const double*, double)
+#define WMX_FLUX_ILP_LIST(PFX)                                                   \
+  PFX template __global__ void k_flux1<0> WMX_FLUX_SIG;                          \
+  PFX template __global__ void k_flux1<1> WMX_FLUX_SIG;                          \
+  PFX template __global__ void k_flux1<2> WMX_FLUX_SIG;                          \
+  PFX template __global__ void k_flux1<3> WMX_FLUX_SIG;                          \
+  PFX template __global__ void k_flux2<float, 0, SC_HYB_KEP4_ALBADA> WMX_FLUX_SIG; \
+  PFX template __global__ void k_flux2<float, 1, SC_HYB_KEP4_ALBADA> WMX_FLUX_SIG; \
+  PFX template __global__ void k_flux2<float, 2, SC_HYB_KEP4_ALBADA> WMX_FLUX_SIG; \
+  PFX template __global__ void k_flux2<float, 3, SC_HYB_KEP4_ALBADA> WMX_FLUX_SIG;
+
+#ifdef WMX_FLUX_ILP_UNIT
+WMX_FLUX_ILP_LIST()
+#else
+WMX_FLUX_ILP_LIST(extern)
 
 
 hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles, int ci_len,
@@ -824,5 +835,6 @@ hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, in
   }
   return hipGetLastError();
 }
+#endif
 
 }  // namespace wmx
