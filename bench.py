@@ -41,9 +41,13 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E, datasheet
-# algorithmic bytes per interior cell per RK3 step: FP64 state 40 words + FP64
-# geometry 13 words x 3 stages; the fused flux kernel moves one third per launch.
-B_ALG_STEP = {"fp64": 632.0, "fp32": 632.0}   # S kept in FP64 in both forms
+# Algorithmic bytes per interior cell per RK3 step, as the flux kernels read them.
+# State (FP64, both forms): stage 1 reads q and writes q1 (10 words), stages 2 and 3
+# read q_k and q0 and write the result (15 words each): 40 words = 320 B.
+# Geometry per stage: FP64 plain form 3 owned face vectors (9 words) + xc (3) + vol (1)
+# = 104 B; FP32-bulk form float face vectors (36 B) + FP64 xc (24 B) + FP64 vol (8 B)
+# = 68 B.  The fused flux kernel moves one third per launch.
+B_ALG_STEP = {"fp64": 320.0 + 3 * 104.0, "fp32": 320.0 + 3 * 68.0}   # 632 / 524 B
 
 
 def make_case(nx, ny, nz, ns):
@@ -54,8 +58,23 @@ def make_case(nx, ny, nz, ns):
     return spec, mesh, solver
 
 
-def cpu_baseline(sample=(32, 128, 128), steps=1):
-    """Oracle (NumPy FP64, single-threaded) on a 32x128x128 block of the same numerics."""
+def host_threads():
+    """Host cores this process may run on, OMP_NUM_THREADS and the BLAS pools NumPy uses."""
+    info = {"affinity_cores": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        from threadpoolctl import threadpool_info
+        info["blas_threads"] = {p.get("internal_api", "?"): p.get("num_threads") for p in threadpool_info()}
+    except Exception as e:                # threadpoolctl absent or failing: say so
+        info["blas_threads"] = f"unknown ({type(e).__name__})"
+    return info
+
+
+def cpu_baseline(sample=(32, 128, 128), steps=4):
+    """Oracle (NumPy FP64) on a 32x128x128 block of the same numerics.  The oracle's
+    array arithmetic is NumPy element-wise code, which runs on one thread whatever the
+    BLAS pool size: `cores` is 1, and the host's core count and thread settings are
+    reported beside it."""
     import cases
     import oracle
     spec, mesh, s = make_case(*sample, oracle)
@@ -68,23 +87,27 @@ def cpu_baseline(sample=(32, 128, 128), steps=1):
     el = time.perf_counter() - t0
     cells = sample[0] * sample[1] * sample[2]
     return {"value": round(cells * steps / el / 1e6, 5), "unit": "Mcell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{steps} step(s) (max_dt + advance) of the same numerics on a "
-                      f"{sample[0]}x{sample[1]}x{sample[2]} channel block, NumPy FP64 oracle, "
-                      f"{el:.1f} s on the host"}
+            "kind": "port", "host": host_threads(),
+            "sample": f"{steps} steps (max_dt + advance) of the same numerics on a "
+                      f"{sample[0]}x{sample[1]}x{sample[2]} channel block, NumPy FP64 oracle "
+                      f"(element-wise, one thread), {el:.1f} s on the host"}
 
 
-def traffic_from_profile(workload):
+def profile_record(workload):
+    """The committed rocprofv3 figures of the flux kernel for `workload` (HBM bytes per
+    launch from FETCH_SIZE / WRITE_SIZE, average launch time from the kernel trace), or
+    {} when none matches."""
     path = os.path.join(ROOT, "profiles", "flux_traffic.json")
     if not os.path.exists(path):
-        return None
+        return {}
     try:
-        rec = json.load(open(path))
+        recs = json.load(open(path))
     except ValueError:
-        return None
-    if rec.get("workload") != workload:
-        return None
-    return rec.get("hbm_bytes_per_launch")
+        return {}
+    for rec in recs if isinstance(recs, list) else [recs]:
+        if rec.get("workload") == workload:
+            return rec
+    return {}
 
 
 def run_form(wmx, form, size, steps, warmup, breakdown, graph=False):
@@ -138,12 +161,24 @@ def form_errors(r64, r32):
             "q_w": float(abs(qw(st32) - qw(st64)) / abs(qw(st64)))}
 
 
-def _roofline(avg_launch_s, b_launch, workload):
+def _roofline(avg_launch_s, b_launch, workload, b_cell_step):
+    """achieved = algorithmic bytes per flux launch / the launch time measured live with
+    HIP events on the launch stream.  The committed rocprofv3 trace's average launch
+    time for the same workload, and the fraction it gives, are reported beside it."""
     achieved = b_launch / avg_launch_s / 1e9
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(workload),
-            "kernel": "k_flux (fused face fluxes + divergence + RK stage)",
-            "alg_bytes_per_launch": b_launch, "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+    prof = profile_record(workload)
+    rec = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("hbm_bytes_per_launch"),
+           "kernel": "k_flux (fused face fluxes + divergence + RK stage)",
+           "alg_bytes_per_cell_step": b_cell_step,
+           "alg_bytes_per_launch": round(b_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
+    us = prof.get("rocprof_avg_launch_us")
+    if us:
+        rec["rocprof_avg_launch_ms"] = round(us / 1e3, 4)
+        rec["rocprof_frac"] = round(b_launch / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+    if rec["traffic"]:
+        rec["traffic_over_alg"] = round(rec["traffic"] / b_launch, 3)
+    return rec
 
 
 def _record(args, value, ms_step, dtype, workload, cfg, roofline):
@@ -173,7 +208,7 @@ def bench_channel(args, wmx):
                                      "constraint; the CFL min taken in the first stage's primitive pass)",
                              "graph": bool(args.graph),
                              "form": head},
-                  _roofline(flux_ms / flux_n / 1e3, b_launch, workload))
+                  _roofline(flux_ms / flux_n / 1e3, b_launch, workload, B_ALG_STEP[head]))
     rec["step_hbm_frac"] = round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
     ci, order, ntiles = r["solver"].tiling()
     rec["config"]["tiling"] = {"ci_len": ci, "xcd_order": order, "tiles": ntiles}
@@ -226,7 +261,8 @@ def bench_ramp(args, wmx):
     flux_ms, flux_n = prof["flux"]
     rec = _record(args, cells * args.steps / el / 1e6, el * 1e3 / args.steps, "f32-bulk+f64", workload,
                   {"cells": cells, "blocks": 1, "ng": 3, "form": "fp32"},
-                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload,
+                            B_ALG_STEP["fp32"]))
     rec["state_finite"] = finite
     return rec
 
@@ -260,10 +296,10 @@ def bench_ensemble(args, wmx):
     prof = ens.profile_read()
     ens.profile(())
     # the same cases stepped one after another (the same step form per case), with the
-    # ensemble's tiling and FP32-bulk gauge so the results can be compared bitwise
+    # ensemble's tiling (each case keeps its own FP32-bulk gauge, as in the ensemble) so
+    # the results can be compared bitwise
     for s in solvers:
         s.set_tiling(*best)
-        s.set_gauge(*ens._gauge)
 
     def seq_step(Qs):
         return [s.step(Q, graph=args.graph)[0] for s, Q in zip(solvers, Qs)]
@@ -275,7 +311,8 @@ def bench_ensemble(args, wmx):
     rec = _record(args, cells * args.steps / el_e / 1e6, el_e * 1e3 / args.steps, "f32-bulk+f64", workload,
                   {"cells": cells, "cases": 16, "blocks": nblk, "form": "fp32",
                    "tiling": {"ci_len": best[0], "xcd_order": best[1]}},
-                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload,
+                            B_ALG_STEP["fp32"]))
     rec["sequential_value"] = round(cells * args.steps / el_s / 1e6, 2)
     rec["ensemble_speedup_vs_sequential"] = round(el_s / el_e, 3)
     rec["ensemble_equals_sequential"] = same
@@ -322,6 +359,23 @@ def bench_wm(args, wmx):
 
     el_n, Qn, prof_n, st_n = run(None)
     el_l, Ql, prof_l, st_l = run(model)
+    # The network's own accuracy (raw outputs, mode 1) and the shipped model's (network
+    # start + Newton polish, mode 0) against the Newton model, on 1e5 points spread over
+    # the trained range (log-uniform Re_h in [1e2, 1e6], T_h/T_w in [0.8, 1.25]).
+    gas = wmx.Gas()
+    rng = np.random.default_rng(7)
+    npts, T_w, p_w, U = 100_000, 300.0, 1e5, 10.0
+    Reh = np.exp(rng.uniform(np.log(1e2), np.log(1e6), npts))
+    rho_w, mu_w = p_w / (gas.R * T_w), float(gas.mu(T_w))
+    h_t = torch.tensor(Reh * mu_w / (U * rho_w), device="cuda")
+    Th_t = torch.tensor(T_w * rng.uniform(0.8, 1.25, npts), device="cuda")
+    tau_n, qw_n = wmx.EquilibriumWallModel()(gas, h_t, U, Th_t, p_w, T_w)
+    net_err = {}
+    for name, mode in (("raw_network", 1), ("polished", 0)):
+        tau_m, qw_m, _ = model.eval(gas, h_t, U, Th_t, p_w, T_wall=T_w, mode=mode)
+        ut_err = torch.abs(torch.sqrt(tau_m / tau_n) - 1.0)
+        net_err[name] = {"u_tau_rel_err_max": float(ut_err.max()), "u_tau_rel_err_mean": float(ut_err.mean()),
+                         "q_w_err_max_over_max_q_w": float(torch.abs(qw_m - qw_n).max() / torch.abs(qw_n).max())}
     n_faces = sum(int(v[0].size) for v in st_l.values())
     model.fallbacks.zero_()
     s.wall_stats(Q0)                      # one evaluation of every wall face
@@ -339,7 +393,8 @@ def bench_wm(args, wmx):
                   {"cells": cells, "blocks": 1, "form": "fp32", "wall_faces": n_faces,
                    "network": "3-64-64-2 ISRU, BF16 MFMA inference, 3 Newton polish steps",
                    "n_match": n_match},
-                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload))
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload,
+                            B_ALG_STEP["fp32"]))
     # FP32-bulk plans evaluate the Newton faces inside the gradient launch (no launch of
     # their own); the learned model keeps its wall launch.  Both step times are reported.
     per = lambda pr: round(pr["wall"][0] / pr["wall"][1], 5) if "wall" in pr else None
@@ -349,6 +404,7 @@ def bench_wm(args, wmx):
         "learned_ms_per_launch": per(prof_l),
         "step_speedup_learned_vs_newton": round(el_n / el_l, 3),
         "fallback_fraction": round(fb / max(1, n_faces), 6), "n_match": n_match,
+        "accuracy_vs_newton_1e5_points": net_err,
         "tau_w_max_rel_err_same_state": d_tau,
         "u_tau_rel_err_after_steps": abs(ut(st_ln) - ut(st_nn)) / ut(st_nn),
         "newton_value": round(cells * args.steps / el_n / 1e6, 2),

@@ -167,8 +167,13 @@ int wmx_plan_fill_passes(const wmx_plan* plan, int32_t* npasses);
    FP32 records hold p - p_ref, T - T_ref and e_int - R T_ref / (gamma - 1); the
    fluxes add the shift's contribution back in FP64.  Values near the flow's mean
    pressure and temperature keep the low-Mach cancellation in FP64.  Ignored by the
-   FP64 plain form. */
+   FP64 plain form.  wmx_plan_set_gauge sets every case of the plan;
+   wmx_plan_set_case_gauge the blocks of one ensemble case (case_id -1: all), so
+   cases with different mean states each keep their own FP32 range.  Both synchronise
+   the device when a workspace is bound (the values live in the device block
+   descriptors, which captured step graphs read at replay). */
 int wmx_plan_set_gauge(wmx_plan* plan, double p_ref, double T_ref);
+int wmx_plan_set_case_gauge(wmx_plan* plan, int32_t case_id, double p_ref, double T_ref);
 
 /* Kernel-class event timing (measurement aid, no reference counterpart).  Classes:
    0 flux+RK, 1 gradient/SGS, 2 primitives, 3 ghost fill, 4 wall model, 5 CFL dt,

@@ -168,6 +168,8 @@ class Solver:
     def wall_loads(self, Q, moment_center=(0.0, 0.0, 0.0)):
         """Integrated wall force and moment per WallModel face (solver.py:181-232)."""
         center = np.asarray(moment_center, dtype=float)
+        if center.shape != (3,) or not np.all(np.isfinite(center)):     # solver.py:187-189
+            raise ValueError("moment_center must contain three finite coordinates")
         out = {}
         for blk, q, g in zip(self.blocks, Q, self.geoms):
             w, T = self.gas.cons_to_prim(q)

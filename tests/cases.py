@@ -104,11 +104,11 @@ def channel_props(Re_b=2.0e4, Ma_b=0.3, T0=300.0, rho0=1.2):
 
 
 def channel_spec(nblocks_x, nx, ny, nz, scheme, sgs, T_wall=300.0, Re_b=2.0e4, beta=2.0,
-                 name="channel", mass_flux=True, splits=None, n_match=1):
+                 name="channel", mass_flux=True, splits=None, n_match=1, rho0=1.2):
     """Wall-modelled channel 2 pi d x 2 d x pi d, `nblocks_x` blocks in x (periodic
     x/z, WallModel on jmin/jmax sampling the `n_match`-th cell off the wall, constant mass
-    flux).  `nx` is the total x count."""
-    pr = channel_props(Re_b=Re_b)
+    flux).  `nx` is the total x count; `rho0` sets the bulk density (p0 = rho0 R T0)."""
+    pr = channel_props(Re_b=Re_b, rho0=rho0)
     d = pr["d"]
     Lx, Lz = 2 * np.pi * d, np.pi * d
     if splits is None:

@@ -87,6 +87,7 @@ def _load():
         "wmx_plan_fill_passes": ([P, C.POINTER(C.c_int32)], C.c_int),
         "wmx_plan_profile": ([P, C.c_int32], C.c_int),
         "wmx_plan_set_gauge": ([P, C.c_double, C.c_double], C.c_int),
+        "wmx_plan_set_case_gauge": ([P, C.c_int32, C.c_double, C.c_double], C.c_int),
         "wmx_plan_profile_read": ([P, C.POINTER(C.c_double), C.POINTER(C.c_int32)], C.c_int),
         "wmx_fill_nodes": ([C.c_int32, C.POINTER(BlockDesc), PP, C.c_int32, C.c_int32, P], C.c_int),
         "wmx_compute_geometry": ([P, C.POINTER(C.c_int32), P, P, P, P, P, P, P, P, P], C.c_int),

@@ -1,6 +1,7 @@
 // C ABI of libwmles_amd.so: plan construction (ghost-sweep phases, tile tables,
 // workspace layout) and the launch sequences of the explicit step.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -398,9 +399,6 @@ NumDev make_numdev(const wmx_numerics_desc* nm) {
   nd.n_sponge = nm->n_sponge;
   std::memcpy(nd.sponge, nm->sponge, sizeof(nd.sponge));
   nd.prec = nm->precision;
-  nd.p_ref = 0.0;
-  nd.T_gauge = 0.0;
-  nd.e_ref = 0.0;
   return nd;
 }
 
@@ -875,14 +873,27 @@ int wmx_plan_fill_passes(const wmx_plan* p, int32_t* npasses) {
   return 0;
 }
 
-int wmx_plan_set_gauge(wmx_plan* p, double p_ref, double T_ref) {
+int wmx_plan_set_case_gauge(wmx_plan* p, int32_t case_id, double p_ref, double T_ref) {
   if (!p) return fail("null plan");
-  if (!(p_ref >= 0.0) || !(T_ref >= 0.0)) return fail("gauge values must be finite and >= 0");
-  p->nd.p_ref = p_ref;
-  p->nd.T_gauge = T_ref;
-  p->nd.e_ref = p->nd.gas.R * T_ref / (p->nd.gas.gamma - 1.0);
-  drop_graphs(p);   // captured launches hold the old values
+  if (!(p_ref >= 0.0) || !(T_ref >= 0.0) || !(p_ref < HUGE_VAL) || !(T_ref < HUGE_VAL))
+    return fail("gauge values must be finite and >= 0");
+  if (case_id < -1 || case_id >= p->n_cases) return fail("case_id out of range");
+  for (BlockDev& B : p->hb)
+    if (case_id < 0 || B.case_id == case_id) {
+      B.p_ref = p_ref;
+      B.T_gauge = T_ref;
+      B.e_ref = p->nd.gas.R * T_ref / (p->nd.gas.gamma - 1.0);
+    }
+  if (p->ws) {
+    // the descriptors are read by kernels that may still be in flight on any stream
+    WMX_CHECK(hipDeviceSynchronize());
+    WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
+  }
   return 0;
+}
+
+int wmx_plan_set_gauge(wmx_plan* p, double p_ref, double T_ref) {
+  return wmx_plan_set_case_gauge(p, -1, p_ref, T_ref);
 }
 
 int wmx_plan_set_wall_net(wmx_plan* p, const wmx_wall_net_desc* net) {

@@ -49,6 +49,9 @@ struct BlockDev {
   long long shell_cells;
   int zfuse;                // bit s: the k ghosts of side s are the sweep's last phase, which
                             // the primitive kernels of a step evaluate themselves
+  // FP32-bulk gauge of the block's case: p' = p - p_ref, T' = T - T_gauge,
+  // e' = e_int - e_ref with e_ref = R T_gauge / (gamma - 1)
+  double p_ref, T_gauge, e_ref;
   wmx_face_desc faces[6];
 };
 
@@ -74,9 +77,7 @@ struct NumDev {
   double cfl;
   int n_sponge;
   double sponge[4][5];
-  int prec;                 // WMX_PREC_FP64 / WMX_PREC_FP32_BULK
-  double p_ref, T_gauge, e_ref;   // FP32-bulk gauge: p' = p - p_ref, T' = T - T_gauge,
-                                  // e' = e_int - e_ref with e_ref = R T_gauge / (gamma - 1)
+  int prec;                 // WMX_PREC_FP64 / WMX_PREC_FP32_BULK (gauge: BlockDev)
   WallNetDev net;
 };
 

@@ -120,7 +120,8 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
                                               const CellWT<float>& R2, float thL, float thR,
                                               const CellGT<float>& gL, const CellGT<float>& gR,
                                               float TL, float TR, const double* xL, const double* xR,
-                                              float sx, float sy, float sz, float* Ff, float* uw) {
+                                              float sx, float sy, float sz, float pr, float er,
+                                              float* Ff, float* uw) {
   const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
                   : (SC == SC_KEP2 || SC == SC_KEP4) ? WMX_SCHEME_KEP : nd.scheme;
   const int order = SC == SC_KEP2 ? 2 : ((SC == SC_HYB_KEP4_ALBADA || SC == SC_KEP4) ? 4 : nd.order);
@@ -138,9 +139,7 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
 #pragma unroll
   for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v];
   if (s > 0.f) {
-    muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz,
-                           float(nd.p_ref), Fu);
-    const float pr = float(nd.p_ref), er = float(nd.e_ref);
+    muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz, pr, Fu);
     Ff[0] = Ff[0] + s * Fu[0];
     Ff[1] = Ff[1] + s * (Fu[1] - pr * sx);
     Ff[2] = Ff[2] + s * (Fu[2] - pr * sy);
@@ -165,7 +164,6 @@ __device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2,
   static_assert(sizeof(R) == 8, "the FP32-bulk face flux is flux_core_f32");
   const R sx = R(Sx), sy = R(Sy), sz = R(Sz);
   const int sch = nd.scheme;
-  const bool gauge = sizeof(R) == 4;
   double Fc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Fu[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double s = sch == WMX_SCHEME_MUSCLHLLC ? 1.0 : 0.0;
   if (sch == WMX_SCHEME_HYBRID) {
@@ -179,16 +177,10 @@ __device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2,
     else central_flux<false, R>(nd.order, L2, L, Rc, R2, sx, sy, sz, Fr, &un4);
 #pragma unroll
     for (int v = 0; v < 5; ++v) Fc[v] = (double)Fr[v];
-    if (gauge) {
-      Fc[1] += nd.p_ref * Sx;
-      Fc[2] += nd.p_ref * Sy;
-      Fc[3] += nd.p_ref * Sz;
-      Fc[4] += nd.e_ref * Fc[0] + nd.p_ref * (double)un4;
-    }
   }
   if (s > 0.0) {
     R Fr[5];
-    muscl_hllc_flux<R>(R(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz, R(nd.p_ref), Fr);
+    muscl_hllc_flux<R>(R(nd.gas.gamma), nd.limiter, L2, L, Rc, R2, sx, sy, sz, R(0), Fr);
 #pragma unroll
     for (int v = 0; v < 5; ++v) Fu[v] = (double)Fr[v];
   }
@@ -491,7 +483,8 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
           const unsigned fi = (unsigned)fidx(N, d, fI, fJ, fK), fc = (unsigned)fcount(N, d);
           const float* S = B.Sf[d];
           flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
-                        ldu(S + fc, fi), ldu(S + 2 * fc, fi), F, &F[5]);
+                            ldu(S + fc, fi), ldu(S + 2 * fc, fi), float(B.p_ref), float(B.e_ref), F,
+                            &F[5]);
         }
         if (task == 2) {
           const int hs = first ? 0 : lo ^ 1;
@@ -533,7 +526,7 @@ __global__ __launch_bounds__(256, 2) void k_flux2(const BlockDev* __restrict__ b
         t = t - ((double)sFk[v][kup] - (double)sFk[v][tid]);
         a[v] = t;
       }
-      a[4] = (a[4] + nd.e_ref * a[0]) + nd.p_ref * a[5];
+      a[4] = (a[4] + B.e_ref * a[0]) + B.p_ref * a[5];
       double r[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) r[v] = a[v] * ivol;

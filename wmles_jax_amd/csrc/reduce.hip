@@ -283,10 +283,10 @@ __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blk
     double qv[5];
     prim_state(blks, b, c, q, gas, zfuse, qv);
     const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
-    st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - nd.T_gauge)));
+    st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - B.T_gauge)));
     st4(B.Pf + 4 * ((size_t)cells + c),
-        make_float4((float)w.r, (float)(w.p - nd.p_ref), (float)mu_suth(gas, w.T),
-                    (float)(w.p / ((gas.gamma - 1.0) * w.r) - nd.e_ref)));
+        make_float4((float)w.r, (float)(w.p - B.p_ref), (float)mu_suth(gas, w.T),
+                    (float)(w.p / ((gas.gamma - 1.0) * w.r) - B.e_ref)));
     int I, J, K;
     if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
   }

@@ -164,7 +164,7 @@ __device__ __forceinline__ void wall_face(const BlockDev* __restrict__ blks, con
   const long long nw = (long long)B.n[t1] * B.n[t2];
   // FP32-bulk plans take the wall flux in gauge form: (p - p_ref) instead of p (the
   // p_ref S terms of a cell's faces telescope; no mass flux, so no energy gauge term)
-  const double pw = nd.prec == WMX_PREC_FP32_BULK ? wb.p - nd.p_ref : wb.p;
+  const double pw = nd.prec == WMX_PREC_FP32_BULK ? wb.p - B.p_ref : wb.p;
   W[idx] = sg * 0.0;
   for (int x = 0; x < 3; ++x) W[(1 + x) * nw + idx] = sg * ((pw * no[x] + tau * (up[x] / U)) * Sm);
   W[4 * nw + idx] = sg * ((-qw) * Sm);

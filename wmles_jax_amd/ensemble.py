@@ -83,7 +83,7 @@ class EnsembleSolver:
         _lib.check(_lib.lib.wmx_plan_bind_workspace(plan, C.c_void_p(self._ws.data_ptr()), nbytes.value))
         self._stream = torch.cuda.Stream(device=self._dev)
         self._dt_dev = torch.empty(16, dtype=torch.float64, device=self._dev)
-        self._gauge = None
+        self._gauge = None            # per-case (p_ref, T_ref) once set
         from .learned_wm import LearnedWallModel
         if isinstance(self.wall_model, LearnedWallModel):
             _lib.check(_lib.lib.wmx_plan_set_wall_net(self._plan, C.byref(self.wall_model.desc())))
@@ -120,12 +120,16 @@ class EnsembleSolver:
         return out
 
     def _ensure_gauge(self, Qs):
+        """FP32-bulk records of case c hold p - p_ref(c), T - T_ref(c): each case's gauge
+        is its own solver's (set_gauge, or the interior means of its first state)."""
         if self.precision != "fp32" or self._gauge is not None:
             return
-        s0 = self.solvers[0]
-        s0._ensure_gauge(s0._as_state(Qs[0]))
-        self._gauge = s0._gauge
-        _lib.check(_lib.lib.wmx_plan_set_gauge(self._plan, *self._gauge))
+        gauges = []
+        for c, (s, Q) in enumerate(zip(self.solvers, Qs)):
+            s._ensure_gauge(s._as_state(Q))
+            gauges.append(s._gauge)
+            _lib.check(_lib.lib.wmx_plan_set_case_gauge(self._plan, c, *s._gauge))
+        self._gauge = gauges
 
     def init_state(self, prim_fns):
         return [s.init_state(f) for s, f in zip(self.solvers, prim_fns)]
