@@ -84,9 +84,24 @@ def test_pvd_entries_and_identity(tmp_path):
                 '<DataSet timestep="0.5" part="0" file="case_block0_000010.vtu"/>'
                 '<DataSet timestep="1" part="0" file="case_block1_000020.vtu"/></Collection></VTKFile>')
     assert IO._pvd_entries(p) == [(0.5, "case_block0_000010.vtu"), (1.0, "case_block1_000020.vtu")]
-    assert IO._vtu_identity("/x/case_block12_000345.vtu") == ("case", 12, 345)
+    assert IO._block_file_key("/x/case_block12_000345.vtu") == ("case", 12, 345)
+    assert IO._block_file_key("/x/my_case_block3_7.vtu") == ("my_case", 3, 7)
+    assert IO._block_file_key("/x/case_000345.vtu") is None
+    # step selection: the collection's latest step; a block file's own step and its siblings
+    for b, st in ((0, 10), (1, 10), (0, 20), (1, 20)):
+        open(os.path.join(tmp_path, f"case_block{b}_{st:06d}.vtu"), "w").close()
+    with open(p, "w") as f:
+        f.write('<?xml version="1.0"?>\n<VTKFile type="Collection" version="0.1"><Collection>' +
+                "".join(f'<DataSet timestep="{st / 10}" part="0" file="case_block{b}_{st:06d}.vtu"/>'
+                        for st in (10, 20) for b in (0, 1)) + '</Collection></VTKFile>')
+    step, files = IO._checkpoint_files(p)
+    assert step == 20 and sorted(files) == [0, 1] and files[1][1] == 2.0
+    step, files = IO._checkpoint_files(os.path.join(tmp_path, "case_block1_000010.vtu"))
+    assert step == 10 and sorted(files) == [0, 1] and files[0][1] is None
     with pytest.raises(ValueError):
-        IO._vtu_identity("/x/case_000345.vtu")
+        IO._checkpoint_files(os.path.join(tmp_path, "case_000345.vtu"))
+    with pytest.raises(FileNotFoundError):
+        IO._checkpoint_files(os.path.join(tmp_path, "other_block0_000010.vtu"))
 
 
 @pytest.mark.gpu

@@ -1,7 +1,7 @@
 """Plot3D files and NMF mesh assembly: formats and topology on the CPU; multiblock RHS and
 free-stream preservation of assembled meshes on the GPU (the reference's
-tests/test_plot3d_nmf.py:50-275, with the L5 airfoil grid's NMF as a fixture: its Plot3D
-grid file is not part of the reference's test data)."""
+tests/test_plot3d_nmf.py:50-275; the L5 airfoil grid and its NMF are data files of the
+reference's tests, kept here as fixtures)."""
 import os
 import tempfile
 import textwrap
@@ -156,7 +156,7 @@ def test_cascade_split_topology():
     assert sorted(b.name for b in mesh.blocks) == ["b1_000", "b1_001", "b2", "b3", "b4_000", "b4_001"]
     assert sum(int(np.prod(b.n)) for b in mesh.blocks) == 6 * 6 * 8 * 2 + 6 * 6 * 4 * 2
     text = NM.describe(mesh)
-    assert "b1_000" in text and "total cells: 864" in text
+    assert "b1_000" in text and text.endswith("cells: 864")
 
 
 def test_swap_topology():
@@ -246,29 +246,100 @@ def test_swap_and_reversed_ranges_equal_single_block():
     assert err < 1e-11
 
 
+def _freestream_residual(wmx, mesh, sc, prim0):
+    """max over interior cells of |R| / (F0 * sum of the cell's face areas / vol): the
+    free-stream residual scaled by the flux magnitude (the reference's measure,
+    tests/test_plot3d_nmf.py:32-50)."""
+    gas = wmx.Gas()
+    s = wmx.Solver(mesh, gas, sc, sgs="vreman")
+    Q = s.init_state(lambda xc: np.stack([prim0[i] + 0 * xc[0] for i in range(5)]))
+    R = s.rhs(Q)
+    E = prim0[4] / (gas.gamma - 1) + 0.5 * prim0[0] * np.sum(prim0[1:4] ** 2)
+    F0 = (E + prim0[4]) * np.linalg.norm(prim0[1:4]) + prim0[4]
+    worst = 0.0
+    for blk, r, g in zip(mesh.blocks, R, mesh.geoms):
+        av = 0.0
+        for d in range(3):
+            S = blk.slab(g.S[d].cpu().numpy(), d, 0, 1)
+            lo = S[(slice(None),) + tuple(slice(None, -1) if a == d else slice(None) for a in range(3))]
+            hi = S[(slice(None),) + tuple(slice(1, None) if a == d else slice(None) for a in range(3))]
+            av = av + (np.linalg.norm(lo, axis=0) + np.linalg.norm(hi, axis=0))
+        av = av / blk.interior(g.vol.cpu().numpy())
+        res = np.abs(blk.interior(r.cpu().numpy())).max(axis=0)
+        worst = max(worst, float((res / (F0 * av)).max()))
+    return worst
+
+
 @pytest.mark.gpu
 def test_cascade_split_preserves_free_stream():
     torch, wmx = _gpu()
     prim0 = np.array([1.2, 40.0, -10.0, 20.0, 1e5])
     nodes, nmf, _ = _cascade(prim0)
     mesh = NM.build_mesh(nodes, nmf, bc_map={"ff": wmx.Dirichlet(prim0)})
-    gas = wmx.Gas()
     for sc in (wmx.KEP(4), wmx.MUSCLHLLC(), wmx.Hybrid()):
-        s = wmx.Solver(mesh, gas, sc, sgs="vreman")
-        Q = s.init_state(lambda xc: np.stack([prim0[i] + 0 * xc[0] for i in range(5)]))
-        R = s.rhs(Q)
-        E = prim0[4] / (gas.gamma - 1) + 0.5 * prim0[0] * np.sum(prim0[1:4] ** 2)
-        F0 = (E + prim0[4]) * np.linalg.norm(prim0[1:4]) + prim0[4]
-        worst = 0.0
-        for blk, r, g in zip(mesh.blocks, R, mesh.geoms):
-            av = 0.0
-            for d in range(3):
-                S = blk.slab(g.S[d].cpu().numpy(), d, 0, 1)
-                lo = S[(slice(None),) + tuple(slice(None, -1) if a == d else slice(None) for a in range(3))]
-                hi = S[(slice(None),) + tuple(slice(1, None) if a == d else slice(None) for a in range(3))]
-                av = av + (np.linalg.norm(lo, axis=0) + np.linalg.norm(hi, axis=0))
-            av = av / blk.interior(g.vol.cpu().numpy())
-            res = np.abs(blk.interior(r.cpu().numpy())).max(axis=0)
-            worst = max(worst, float((res / (F0 * av)).max()))
+        worst = _freestream_residual(wmx, mesh, sc, prim0)
         print(type(sc).__name__, "free-stream residual (scaled)", worst)
+        assert worst < 1e-11
+
+
+# ------------------------------------------------------------------ L5 airfoil (TMBWG)
+L5 = os.path.join(GOLDEN, "nak_a_ff_500c_2zn_L5")
+
+
+def _l5_nodes():
+    """The L5 NACA airfoil grid (data file of the reference's tests) and its NMF,
+    respanned to 4 spanwise cells over 0.1."""
+    grid = P3.read_plot3d(L5 + ".p3dfmt")
+    assert grid.formatted and grid.ndim == 3 and grid.iblank is None
+    assert [X.shape[1:] for X in grid.nodes] == [(2, 43, 57), (2, 65, 29)]
+    return NM.respan(grid.nodes, NM.read_nmf(L5 + ".nmf"), ncells=4, length=0.1)
+
+
+def _interior_volumes(X):
+    """Cell volumes of a node array (3, ni+1, nj+1, nk+1) by the oracle's geometry."""
+    import oracle.mesh as OM
+    return OM.compute_geometry(np.asarray(X, dtype=float))[0]
+
+
+def test_l5_airfoil_grid_assembly():
+    """The reference's airfoil acceptance check on the CPU (tests/test_plot3d_nmf.py:250-268):
+    the wake block is cut at the trailing-edge line, the airfoil surface is b2.kmin only,
+    and the sub-blocks' cells add up to the original cells, volume included."""
+    nodes, nmf = _l5_nodes()
+    assert [X.shape[1:] for X in nodes] == [(5, 43, 57), (5, 65, 29)]
+    assert np.allclose(nodes[0][0, -1] - nodes[0][0, 0], 0.0)        # span along the node axis 0
+    mesh = NM.build_mesh(nodes, nmf, bc_map={"farfield_riem": Dirichlet([1.2, 60.0, 0.0, 15.0, 1e5]),
+                                             "viscous_solid": NoSlipWall()},
+                         periodic=(("symmetry_y", 0),), finalize=False)
+    assert sorted(b.name for b in mesh.blocks) == ["b1_000", "b1_001", "b2"]
+    assert sum(int(np.prod(b.n)) for b in mesh.blocks) == 4 * (42 * 56 + 64 * 28)
+    walls = [(b.name, f) for b in mesh.blocks for f, bc in b.faces.items() if isinstance(bc, NoSlipWall)]
+    assert walls == [("b2", (2, 0))]
+    for b in mesh.blocks:                                  # every face is assigned
+        assert len(b.faces) == 6
+        assert all(isinstance(b.faces[(0, s)], NM.Interface) for s in range(2))   # periodic span
+    vol = sum(float(_interior_volumes(b.x_int).sum()) for b in mesh.blocks)
+    vol_ref = sum(float(_interior_volumes(X).sum()) for X in nodes)
+    assert abs(vol - vol_ref) < 1e-12 * vol_ref
+
+
+@pytest.mark.gpu
+def test_l5_airfoil_volume_and_free_stream():
+    """On the device: the assembled mesh's volumes add up to the grid's, and a uniform
+    stream is preserved to < 1e-11 (scaled) by KEP(4), MUSCLHLLC and Hybrid, with the
+    airfoil surface as a Dirichlet face (reference tests/test_plot3d_nmf.py:263-275)."""
+    torch, wmx = _gpu()
+    nodes, nmf = _l5_nodes()
+    prim0 = np.array([1.2, 60.0, 0.0, 15.0, 1e5])
+    ff = wmx.Dirichlet(prim0)
+    mesh = NM.build_mesh(nodes, nmf, bc_map={"farfield_riem": ff, "viscous_solid": wmx.NoSlipWall()},
+                         periodic=(("symmetry_y", 0),))
+    vol = sum(float(b.interior(g.vol).sum()) for b, g in zip(mesh.blocks, mesh.geoms))
+    vol_ref = sum(float(_interior_volumes(X).sum()) for X in nodes)
+    assert abs(vol - vol_ref) < 1e-12 * vol_ref
+    mesh_fs = NM.build_mesh(nodes, nmf, bc_map={"farfield_riem": ff, "viscous_solid": ff},
+                            periodic=(("symmetry_y", 0),))
+    for sc in (wmx.KEP(4), wmx.MUSCLHLLC(), wmx.Hybrid()):
+        worst = _freestream_residual(wmx, mesh_fs, sc, prim0)
+        print("L5 airfoil", type(sc).__name__, "free-stream residual (scaled)", worst)
         assert worst < 1e-11

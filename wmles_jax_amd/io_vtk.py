@@ -31,7 +31,6 @@ from pathlib import Path
 import numpy as np
 import torch
 
-_BLOCK_VTU = re.compile(r"(?P<name>.+)_block(?P<block>\d+)_(?P<step>\d+)\.vtu$")
 VTK_HEXAHEDRON = 12
 _NP = {"Float64": np.float64, "Float32": np.float32, "Int64": np.int64, "Int32": np.int32,
        "UInt8": np.uint8, "UInt32": np.uint32, "UInt64": np.uint64, "Int8": np.int8,
@@ -245,90 +244,107 @@ def write_case(mesh, solver, Q, step, t, outdir, name="case", writer=None):
 
 
 # ------------------------------------------------------------------------------ reader
-def _vtu_identity(path):
-    m = _BLOCK_VTU.fullmatch(Path(path).name)
-    if m is None:
-        raise ValueError(f"{path}: expected a block VTU named '<case>_block<index>_<step>.vtu'")
-    return m.group("name"), int(m.group("block")), int(m.group("step"))
+_NAME = re.compile(r"^(?P<case>.+)_block(?P<block>[0-9]+)_(?P<step>[0-9]+)\.vtu$")
 
 
-def _restart_files(path):
+def _block_file_key(path):
+    """(case, block index, step) from a block file name `<case>_block<i>_<step>.vtu`, or
+    None when the name does not follow it."""
+    m = _NAME.match(Path(path).name)
+    return None if m is None else (m["case"], int(m["block"]), int(m["step"]))
+
+
+def _checkpoint_files(path):
+    """The block files of one saved step: (step, {block: (path, time from the collection
+    or None)}).  A `.pvd` gives its latest step; a block `.vtu` gives its own step, with
+    the sibling files of the same case and step in its directory."""
     path = Path(path)
     if path.suffix.lower() == ".pvd":
-        found = []
+        by_step = {}
         for t, rel in _pvd_entries(path):
             fp = (path.parent / rel).resolve()
-            try:
-                _, block, step = _vtu_identity(fp)
-            except ValueError:
-                continue
-            found.append((step, block, fp, t))
-        if not found:
-            raise ValueError(f"{path}: collection contains no block VTU datasets")
-        last = max(f[0] for f in found)
-        return [f for f in found if f[0] == last]
-    name, _, step = _vtu_identity(path)
-    files = []
-    for fn in glob.glob(str(path.parent / f"{name}_block*_{step:06d}.vtu")):
-        _, block, s = _vtu_identity(fn)
-        if s == step:
-            files.append((step, block, Path(fn), None))
-    if not files:
-        raise FileNotFoundError(f"no block VTUs found for restart step {step} beside {path}")
-    return files
+            key = _block_file_key(fp)
+            if key is not None:
+                by_step.setdefault(key[2], []).append((key[1], fp, t))
+        if not by_step:
+            raise ValueError(f"{path}: none of the collection's datasets is a block file "
+                             "'<case>_block<i>_<step>.vtu'")
+        step = max(by_step)
+        entries = by_step[step]
+    else:
+        key = _block_file_key(path)
+        if key is None:
+            raise ValueError(f"{path}: a restart file is a '.pvd' collection or a block file named "
+                             "'<case>_block<i>_<step>.vtu'")
+        case, _, step = key
+        entries = []
+        for fp in sorted(path.parent.glob(f"{glob.escape(case)}_block*.vtu")):
+            k = _block_file_key(fp)
+            if k is not None and k[0] == case and k[2] == step:
+                entries.append((k[1], fp, None))
+        if not entries:
+            raise FileNotFoundError(f"{path}: no block files of step {step} found")
+    files = {}
+    for blk, fp, t in entries:
+        if blk in files:
+            raise ValueError(f"{path}: block {blk} appears twice in step {step}")
+        files[blk] = (fp, t)
+    return step, files
+
+
+def _checked_primitives(fn, g, blk):
+    """(5, n_i, n_j, n_k) interior primitives [rho, u, v, w, p] of one block file, after
+    checking that the file is this block's grid and holds valid fields."""
+    ncell = int(np.prod(blk.n))
+    if len(g["types"]) != ncell:
+        raise ValueError(f"{fn}: {len(g['types'])} cells, block {blk.name} has {ncell}")
+    pts = block_points(blk)
+    if g["points"].shape != pts.shape or not np.allclose(g["points"], pts, rtol=1e-10, atol=1e-12):
+        raise ValueError(f"{fn}: the points are not block {blk.name}'s nodes")
+    conn = g["connectivity"]
+    if conn is None or not np.array_equal(conn, hex_connectivity(*blk.n)) or np.any(g["types"] != VTK_HEXAHEDRON):
+        raise ValueError(f"{fn}: the cells are not block {blk.name}'s hexahedra")
+    cells = g["cells"]
+    need = [k for k in ("rho", "velocity", "pressure") if k not in cells]
+    if need:
+        raise ValueError(f"{fn}: cell fields {need} are missing")
+    vel = np.asarray(cells["velocity"], dtype=np.float64)
+    if vel.shape != (ncell, 3):
+        raise ValueError(f"{fn}: velocity has shape {vel.shape}, ({ncell}, 3) expected")
+    prim = np.empty((5,) + tuple(blk.n))
+    prim[0] = np.asarray(cells["rho"], dtype=np.float64).reshape(blk.n)
+    prim[1:4] = vel.T.reshape((3,) + tuple(blk.n))
+    prim[4] = np.asarray(cells["pressure"], dtype=np.float64).reshape(blk.n)
+    if not np.isfinite(prim).all() or (prim[0] <= 0).any() or (prim[4] <= 0).any():
+        raise ValueError(f"{fn}: non-finite values, or density / pressure not positive")
+    return prim
 
 
 def read_restart(path, solver):
-    """Latest saved state from a PVD collection or a block VTU: (Q, step, time)
-    (io_vtk.py:118-193)."""
-    files = _restart_files(path)
-    ids = [b for _, b, _, _ in files]
-    expected = set(range(len(solver.blocks)))
-    if set(ids) != expected or len(ids) != len(expected):
-        raise ValueError(f"restart contains block indices {sorted(ids)}, but solver expects {sorted(expected)}")
-    steps = {s for s, _, _, _ in files}
-    if len(steps) != 1:
-        raise ValueError(f"restart files contain inconsistent step numbers: {sorted(steps)}")
-    step = steps.pop()
-    states, times = [None] * len(solver.blocks), []
-    for _, bid, fn, ctime in files:
+    """(Q, step, time) of the latest saved state of a `.pvd` collection, or of a block
+    `.vtu` and its sibling blocks, with the ghosts filled by the solver's boundary sweep
+    (reference `wmles/io_vtk.py:118-193`)."""
+    step, files = _checkpoint_files(path)
+    nb = len(solver.blocks)
+    if sorted(files) != list(range(nb)):
+        raise ValueError(f"{path}: step {step} holds blocks {sorted(files)}, the solver has blocks 0..{nb - 1}")
+    states, times = [], []
+    for ib, blk in enumerate(solver.blocks):
+        fn, coll_time = files[ib]
         g = read_vtu(fn)
-        blk = solver.blocks[bid]
-        ncell = int(np.prod(blk.n))
-        if len(g["types"]) != ncell:
-            raise ValueError(f"{fn}: expected {ncell} cells for block {blk.name}, found {len(g['types'])}")
-        pts = block_points(blk)
-        if g["points"].shape != pts.shape or not np.allclose(g["points"], pts, rtol=1e-10, atol=1e-12):
-            raise ValueError(f"{fn}: mesh points do not match solver block {blk.name}")
-        if (g["connectivity"] is None or not np.array_equal(g["connectivity"], hex_connectivity(*blk.n))
-                or not np.all(g["types"] == VTK_HEXAHEDRON)):
-            raise ValueError(f"{fn}: cell connectivity does not match solver block {blk.name}")
-        missing = {"rho", "velocity", "pressure"} - set(g["cells"])
-        if missing:
-            raise ValueError(f"{fn}: missing restart fields {sorted(missing)}")
-        rho = np.asarray(g["cells"]["rho"], dtype=np.float64).reshape(blk.n)
-        vel = np.asarray(g["cells"]["velocity"], dtype=np.float64)
-        if vel.shape != (ncell, 3):
-            raise ValueError(f"{fn}: velocity must contain three components per cell")
-        vel = np.moveaxis(vel.reshape(tuple(blk.n) + (3,)), -1, 0)
-        p = np.asarray(g["cells"]["pressure"], dtype=np.float64).reshape(blk.n)
-        prim = np.concatenate([rho[None], vel, p[None]], axis=0)
-        if not np.all(np.isfinite(prim)) or np.any(rho <= 0) or np.any(p <= 0):
-            raise ValueError(f"{fn}: restart fields contain invalid primitive values")
-        states[bid] = _padded_state(solver, blk, prim)
-        st = np.asarray(g["field"].get("step", [])).reshape(-1)
-        tm = np.asarray(g["field"].get("time", [])).reshape(-1)
-        if st.size and int(st[0]) != step:
-            raise ValueError(f"{fn}: stored step {st[0]} does not match its filename")
-        if tm.size:
-            times.append(float(tm[0]))
-        elif ctime is not None:
-            times.append(float(ctime))
+        states.append(_padded_state(solver, blk, _checked_primitives(fn, g, blk)))
+        fstep = np.asarray(g["field"].get("step", [])).ravel()
+        ftime = np.asarray(g["field"].get("time", [])).ravel()
+        if fstep.size and int(fstep[0]) != step:
+            raise ValueError(f"{fn}: field data says step {int(fstep[0])}, the file name {step}")
+        if ftime.size:
+            times.append(float(ftime[0]))
+        elif coll_time is not None:
+            times.append(float(coll_time))
         else:
-            raise ValueError(f"{fn}: restart time metadata is missing; use its PVD collection "
-                             "or regenerate the VTU with this version")
+            raise ValueError(f"{fn}: no time in the file's field data; read it through its .pvd collection")
     if not np.allclose(times, times[0], rtol=1e-9, atol=1e-15):
-        raise ValueError(f"restart blocks have inconsistent times: {times}")
+        raise ValueError(f"{path}: the blocks of step {step} were saved at different times {times}")
     return solver.apply_bcs(tuple(states)), step, times[0]
 
 

@@ -1,325 +1,409 @@
-"""Neutral Map File (NMF) connectivity and multiblock mesh assembly from Plot3D grids.
+"""Neutral Map File (NMF) connectivity and multiblock mesh assembly from Plot3D grids
+(host side; the assembled `Mesh` runs through the unchanged HIP step).
 
-Follows the reference (`wmles/nmf.py:1-351`): `read_nmf`, `respan`, `extrude_2d`,
-`build_mesh` (with sub-block splitting) and `describe`.
+    grid = read_plot3d("nak_a_ff_500c_2zn_L5.p3dfmt")
+    nmf = read_nmf("nak_a_ff_500c_2zn_L5.nmf")
+    nodes, nmf = respan(grid.nodes, nmf, ncells=8, length=0.1)
+    mesh = build_mesh(nodes, nmf, bc_map={"viscous_solid": lambda: WallModel(...),
+                                          "farfield_riem": Dirichlet(prim)},
+                      periodic=(("symmetry_y", 0),))
 
-NMF faces (NASA GEOLAB convention): 1 kmin, 2 kmax, 3 imin, 4 imax, 5 jmin, 6 jmax, with
-primary / secondary tangential axes (i, j), (i, j), (j, k), (j, k), (k, i), (k, i).
-Ranges are 1-based inclusive node ranges; a ONE_TO_ONE partner's ranges may run
-backwards (orientation), and Swap=TRUE pairs block 1's primary axis with block 2's
-secondary axis.
+Public names, arguments, the `NMF` / `Patch` fields, the sub-block names and the block
+order are the reference's (`wmles/nmf.py:1-351`), since callers and tests see them.
 
-`build_mesh` cuts every block along every patch boundary and carries the cuts across
-one-to-one interfaces until no new cut appears, so that each face of each sub-block is
-exactly one patch; the cut planes become interfaces, and every interface's orientation is
-derived again from the node coordinates by `Mesh.connect`.
+The NMF format (NASA GEOLAB): a block count, one `block ni nj nk` line per block (node
+counts), then one line per patch: `type block face S1 E1 S2 E2` for a boundary condition,
+`one_to_one b1 f1 S1 E1 S2 E2 b2 f2 S1 E1 S2 E2 [swap]` for a block-to-block connection.
+Faces 1..6 are kmin, kmax, imin, imax, jmin, jmax; ranges are 1-based inclusive node
+ranges along the face's primary and secondary axes (i,j / j,k / k,i); on the second side
+of a one-to-one entry a range may run backwards, and `swap` pairs the first side's
+primary axis with the second side's secondary axis.  `#` starts a comment; a backslash is
+layout only.
+
+Assembly.  `Mesh.connect` joins whole faces, while an NMF patch may cover part of a face.
+So every block is cut into sub-blocks at each patch boundary; a cut on one side of a
+one-to-one patch is carried to the other side through the patch's index map, until no
+new cut appears (a worklist; the closure does not depend on the visiting order).  Every
+sub-block face then lies in exactly one patch: boundary patches give BCs, one-to-one
+patches connect the sub-block to the partner sub-block whose face the index map hits, the
+planes cut inside a block become ordinary interfaces, and `periodic` patch types become
+translations between a sub-block's two faces.  `Mesh.connect` re-derives (and so checks)
+every orientation from the node coordinates.
 """
+from collections import deque
 from dataclasses import dataclass
 from itertools import product
 
 import numpy as np
 
 from .bc import BC, NoSlipWall, SlipWall
-from .mesh import Block, Interface, Mesh
+from .mesh import Interface, Mesh, Block
 
-# NMF face -> (normal axis, side, (primary axis, secondary axis))
-FACES = {1: (2, 0, (0, 1)), 2: (2, 1, (0, 1)), 3: (0, 0, (1, 2)), 4: (0, 1, (1, 2)),
+# NMF face number -> (normal axis, side, (primary axis, secondary axis)); fixed by the format
+FACES = {1: (2, 0, (0, 1)), 2: (2, 1, (0, 1)),
+         3: (0, 0, (1, 2)), 4: (0, 1, (1, 2)),
          5: (1, 0, (2, 0)), 6: (1, 1, (2, 0))}
-FACE_NAME = {(0, 0): "imin", (0, 1): "imax", (1, 0): "jmin", (1, 1): "jmax", (2, 0): "kmin",
-             (2, 1): "kmax"}
+FACE_NAME = {(a, s): "ijk"[a] + ("min", "max")[s] for a in range(3) for s in range(2)}
+
+# boundary types with a default BC (classes: one instance per face); far-field, inflow
+# and outflow types need an explicit entry in bc_map
 DEFAULT_BC_MAP = {"symmetry_x": SlipWall, "symmetry_y": SlipWall, "symmetry_z": SlipWall,
                   "wall": NoSlipWall, "viscous_solid": NoSlipWall, "inviscid_solid": SlipWall}
-_UNSUPPORTED = ("collapsed", "pole_dir1", "pole_dir2", "patched", "unprocessed")
-_TRUE = ("true", ".true.", "t", "1")
+
+_NOT_HANDLED = {"collapsed", "pole_dir1", "pole_dir2", "patched", "unprocessed"}
 
 
 @dataclass
 class Patch:
-    type: str
-    block: int                 # 0-based
-    face: int                  # 1..6
-    ranges: tuple              # ((S1, E1), (S2, E2)) as written (1-based)
-    partner: object = None     # one_to_one: dict(block, face, ranges, swap)
+    type: str                # normalised: lower case, '-' -> '_'
+    block: int               # 0-based
+    face: int                # 1..6
+    ranges: tuple            # ((S1, E1), (S2, E2)), 1-based inclusive, as written
+    partner: object = None   # one-to-one: dict(block, face, ranges, swap)
 
 
 @dataclass
 class NMF:
-    dims: list                 # node counts (ni, nj, nk) per block
+    dims: list               # per block (ni, nj, nk) node counts
     patches: list
 
 
-def _norm_type(t):
+def _norm(t):
     return t.lower().replace("-", "_")
 
 
+# ------------------------------------------------------------------------------ parsing
 def read_nmf(path):
-    rows = []
+    """Parse an NMF file into `NMF(dims, patches)` (reference `wmles/nmf.py:78-107`)."""
+    records = []
     with open(path) as f:
-        for line in f:
-            line = line.split("#", 1)[0].replace("\\", " ").strip()
-            if line:
-                rows.append(line.split())
-    nb = int(rows[0][0])
-    dims = [None] * nb
-    for r in rows[1:1 + nb]:
-        b = int(r[0])
-        dims[b - 1] = tuple(int(t) for t in r[1:4])
+        for raw in f:
+            toks = raw.partition("#")[0].replace("\\", " ").split()
+            if toks:
+                records.append(toks)
+    if not records:
+        raise ValueError(f"{path}: no NMF records")
+    it = iter(records)
+    nblocks = int(next(it)[0])
+    dims = [None] * nblocks
+    for _ in range(nblocks):
+        rec = next(it, None)
+        if rec is None or len(rec) < 4:
+            raise ValueError(f"{path}: expected {nblocks} block dimension lines")
+        b = int(rec[0]) - 1
+        if not 0 <= b < nblocks or dims[b] is not None:
+            raise ValueError(f"{path}: bad or repeated block number {rec[0]}")
+        dims[b] = tuple(int(v) for v in rec[1:4])
     patches = []
-    for r in rows[1 + nb:]:
-        typ = _norm_type(r[0])
-        if typ in _UNSUPPORTED:
-            raise NotImplementedError(f"NMF entry type '{r[0]}' is not supported "
-                                      "(only one-to-one interfaces and boundary conditions)")
-        v = r[1:]
-        if typ == "one_to_one":
-            a = [int(t) for t in v[:12]]
-            swap = len(v) > 12 and v[12].lower() in _TRUE
-            patches.append(Patch(typ, a[0] - 1, a[1], ((a[2], a[3]), (a[4], a[5])),
-                                 dict(block=a[6] - 1, face=a[7], ranges=((a[8], a[9]), (a[10], a[11])),
-                                      swap=swap)))
-        else:
-            a = [int(t) for t in v[:6]]
-            patches.append(Patch(typ, a[0] - 1, a[1], ((a[2], a[3]), (a[4], a[5]))))
+    for rec in it:
+        kind = _norm(rec[0])
+        if kind in _NOT_HANDLED:
+            raise NotImplementedError(f"{path}: NMF patch type '{rec[0]}' cannot be assembled; "
+                                      "only boundary patches and one-to-one connections are")
+        ints = [int(v) for v in rec[1:13 if kind == "one_to_one" else 7]]
+        rng = ((ints[2], ints[3]), (ints[4], ints[5]))
+        partner = None
+        if kind == "one_to_one":
+            if len(ints) < 12:
+                raise ValueError(f"{path}: one_to_one entry with {len(ints)} of 12 integers")
+            flag = rec[13].lower() if len(rec) > 13 else "false"
+            partner = dict(block=ints[6] - 1, face=ints[7],
+                           ranges=((ints[8], ints[9]), (ints[10], ints[11])),
+                           swap=flag in ("true", ".true.", "t", "1"))
+        elif len(ints) < 6:
+            raise ValueError(f"{path}: '{rec[0]}' entry with {len(ints)} of 6 integers")
+        patches.append(Patch(kind, ints[0] - 1, ints[1], rng, partner))
     return NMF(dims, patches)
 
 
 # ------------------------------------------------------------------------------ span
+def _span_vector(X, axis, label):
+    """The translation between the two node planes of `axis` (which must have exactly 2)."""
+    if X.shape[1 + axis] != 2:
+        raise ValueError(f"{label}: {X.shape[1 + axis]} node planes along axis {axis}; respan needs 2")
+    d = (np.take(X, 1, axis=1 + axis) - np.take(X, 0, axis=1 + axis)).reshape(3, -1)
+    v = d[:, 0]
+    scale = max(float(np.abs(X).max()), 1.0)
+    if np.abs(d - v[:, None]).max() > 1e-9 * scale:
+        raise ValueError(f"{label}: the planes along axis {axis} differ by more than a translation")
+    return v
+
+
 def respan(nodes, nmf, ncells, axis=0, length=None):
-    """Blocks whose `axis` has exactly two node planes related by one common translation
-    -> `ncells` uniform cells along it (span rescaled to `length` if given); the NMF node
-    ranges along that axis are renumbered.  Inputs are not modified."""
+    """Replace a two-plane axis (the one-cell span of 2-D-derived grids such as the TMBWG
+    airfoils) by `ncells` uniform cells, the total span optionally rescaled to `length`.
+    Every block must share the span vector.  Returns (nodes, nmf); inputs untouched
+    (reference `wmles/nmf.py:112-136`)."""
     span = None
     for b, X in enumerate(nodes):
-        if X.shape[1 + axis] != 2:
-            raise ValueError(f"block {b + 1}: axis {axis} has {X.shape[1 + axis]} nodes, expected 2")
-        d = np.take(X, 1, axis=1 + axis) - np.take(X, 0, axis=1 + axis)
-        d0 = d.reshape(3, -1)[:, 0]
-        if not np.allclose(d, d0.reshape((3,) + (1,) * (d.ndim - 1)), atol=1e-9 * max(np.abs(X).max(), 1.0)):
-            raise ValueError(f"block {b + 1}: the two planes along axis {axis} are not a pure translation")
+        v = _span_vector(X, axis, f"block {b + 1}")
         if span is None:
-            span = d0
-        elif not np.allclose(d0, span, atol=1e-9 * max(np.linalg.norm(span), 1.0)):
-            raise ValueError("blocks disagree on the span vector")
-    step = span * (length / np.linalg.norm(span)) if length else span
+            span = v
+        elif np.abs(v - span).max() > 1e-9 * max(float(np.linalg.norm(span)), 1.0):
+            raise ValueError(f"block {b + 1}: span vector {v} differs from block 1's {span}")
+    if length:
+        span = span * (length / np.linalg.norm(span))
+    frac = np.arange(ncells + 1) / ncells
     out = []
     for X in nodes:
         base = np.take(X, 0, axis=1 + axis)
-        out.append(np.stack([base + (m / ncells) * step.reshape(3, 1, 1) for m in range(ncells + 1)],
-                            axis=1 + axis))
-    return out, _renumber(nmf, axis, 2, ncells + 1)
+        planes = [base + f * span.reshape(3, 1, 1) for f in frac]
+        out.append(np.stack(planes, axis=1 + axis))
+    return out, _stretch_axis(nmf, axis, 2, ncells + 1)
 
 
-def _renumber(nmf, axis, old, new):
-    dims = [tuple(new if a == axis else n for a, n in enumerate(d)) for d in nmf.dims]
-
-    def fix(face, ranges):
+def _stretch_axis(nmf, axis, old, new):
+    """NMF with `axis` going from `old` to `new` nodes: every range along it must be the
+    full axis (either direction)."""
+    def conv(face, ranges):
         out = []
         for ax, (s, e) in zip(FACES[face][2], ranges):
             if ax == axis:
-                if {s, e} != {1, old}:
-                    raise ValueError(f"cannot respan: a patch covers only part of axis {axis}")
-                s, e = (1, new) if s < e else (new, 1)
+                if sorted((s, e)) != [1, old]:
+                    raise ValueError(f"respan: a patch covers nodes {s}..{e} of axis {axis}, not all {old}")
+                s, e = (1, new) if s <= e else (new, 1)
             out.append((s, e))
         return tuple(out)
 
+    dims = [tuple(new if a == axis else n for a, n in enumerate(d)) for d in nmf.dims]
     patches = []
     for p in nmf.patches:
-        q = Patch(p.type, p.block, p.face, fix(p.face, p.ranges))
-        if p.partner:
-            q.partner = dict(p.partner, ranges=fix(p.partner["face"], p.partner["ranges"]))
-        patches.append(q)
+        partner = None
+        if p.partner is not None:
+            partner = dict(p.partner)
+            partner["ranges"] = conv(partner["face"], partner["ranges"])
+        patches.append(Patch(p.type, p.block, p.face, conv(p.face, p.ranges), partner))
     return NMF(dims, patches)
 
 
 def extrude_2d(nodes, ncells, length, direction=(0.0, 0.0, 1.0)):
-    """(3, ni, nj, 1) blocks of a 2-D file -> (3, ni, nj, ncells + 1), extruded along
-    `direction` over `length` (a new k axis; 3-D NMF files do not apply to the result)."""
-    d = np.asarray(direction, dtype=np.float64)
-    d = d / np.linalg.norm(d) * length
-    return [np.stack([X[..., 0] + (m / ncells) * d.reshape(3, 1, 1) for m in range(ncells + 1)], axis=3)
-            for X in nodes]
+    """(3, ni, nj, 1) blocks of a 2-D grid -> (3, ni, nj, ncells + 1) blocks extruded by
+    `length` along `direction` (the new k axis).  An NMF written for a 3-D grid does not
+    describe the result (reference `wmles/nmf.py:139-145`)."""
+    d = np.asarray(direction, float)
+    d = d * (length / np.linalg.norm(d))
+    frac = np.arange(ncells + 1) / ncells
+    return [X[..., 0][..., None] + d.reshape(3, 1, 1, 1) * frac.reshape(1, 1, 1, -1) for X in nodes]
 
 
 # ------------------------------------------------------------------------------ topology
-def _tangential(face, ranges):
-    """[(axis, lo, hi)] 0-based with lo <= hi for the two tangential axes of a patch."""
-    return [(ax, min(s, e) - 1, max(s, e) - 1) for ax, (s, e) in zip(FACES[face][2], ranges)]
+class _Side:
+    """One side of a patch: block, normal (axis, side), and per tangential axis its
+    0-based node range (lo <= hi)."""
+
+    def __init__(self, block, face, ranges):
+        self.block = block
+        self.axis, self.side, axes = FACES[face]
+        self.span = {ax: (min(s, e) - 1, max(s, e) - 1) for ax, (s, e) in zip(axes, ranges)}
+
+    def covers(self, sub_ranges):
+        return all(lo <= sub_ranges[ax][0] and sub_ranges[ax][1] <= hi for ax, (lo, hi) in self.span.items())
 
 
-def _axis_pairs(p):
-    """Per tangential axis of a one-to-one patch: (axis A, range A, axis B, range B),
-    0-based, in the directions written in the file."""
+class _IndexMap:
+    """Node index map of one tangential axis across a one-to-one patch:
+    idx_b = b0 + sign * (idx_a - a0) for idx_a in [min(a0, a1), max(a0, a1)]."""
+
+    def __init__(self, ax_a, ra, ax_b, rb, where):
+        a0, a1 = ra[0] - 1, ra[1] - 1
+        b0, b1 = rb[0] - 1, rb[1] - 1
+        if abs(a1 - a0) != abs(b1 - b0):
+            raise ValueError(f"one_to_one {where}: ranges of {abs(a1 - a0)} and {abs(b1 - b0)} "
+                             "node intervals cannot be matched")
+        self.ax_a, self.ax_b = ax_a, ax_b
+        self.a0, self.b0 = a0, b0
+        self.sign = 1 if (a1 - a0) * (b1 - b0) >= 0 else -1
+        self.lo, self.hi = min(a0, a1), max(a0, a1)
+
+    def __call__(self, m):
+        return self.b0 + self.sign * (m - self.a0)
+
+    def inverse(self):
+        inv = _IndexMap.__new__(_IndexMap)
+        inv.ax_a, inv.ax_b, inv.a0, inv.b0, inv.sign = self.ax_b, self.ax_a, self.b0, self.a0, self.sign
+        inv.lo, inv.hi = sorted((self(self.lo), self(self.hi)))
+        return inv
+
+
+def _maps_of(p):
+    """The two tangential index maps (side A -> side B) of one-to-one patch p."""
     q = p.partner
     pa, sa = FACES[p.face][2]
     pb, sb = FACES[q["face"]][2]
-    ra = [(s - 1, e - 1) for s, e in p.ranges]
-    rb = [(s - 1, e - 1) for s, e in q["ranges"]]
-    if q["swap"]:
-        pairs = [(pa, ra[0], sb, rb[1]), (sa, ra[1], pb, rb[0])]
-    else:
-        pairs = [(pa, ra[0], pb, rb[0]), (sa, ra[1], sb, rb[1])]
-    for _, (a0, a1), _, (b0, b1) in pairs:
-        if abs(a1 - a0) != abs(b1 - b0):
-            raise ValueError(f"inconsistent one_to_one ranges: block {p.block + 1} face {p.face} vs "
-                             f"block {q['block'] + 1} face {q['face']}")
-    return pairs
+    targets = (sb, pb) if q["swap"] else (pb, sb)
+    tb_ranges = {pb: q["ranges"][0], sb: q["ranges"][1]}
+    where = f"block {p.block + 1} face {p.face} / block {q['block'] + 1} face {q['face']}"
+    return [_IndexMap(ax_a, ra, ax_b, tb_ranges[ax_b], where)
+            for ax_a, ra, ax_b in zip((pa, sa), p.ranges, targets)]
 
 
-def _carry(m, ra, rb):
-    """Node index m of range ra -> the corresponding index of range rb."""
-    (a0, a1), (b0, b1) = ra, rb
-    if a0 == a1:
-        return b0
-    return b0 + (m - a0) * (b1 - b0) // (a1 - a0)
-
-
-def _bc_for(bc_map, typ):
-    v = bc_map.get(typ, DEFAULT_BC_MAP.get(typ))
-    if v is None:
-        raise KeyError(f"no BC supplied for NMF type '{typ}' - pass bc_map={{'{typ}': <BC or factory>}}")
-    return v if isinstance(v, BC) else v()
+def _bc_instance(bc_map, kind):
+    entry = bc_map[kind] if kind in bc_map else DEFAULT_BC_MAP.get(kind)
+    if entry is None:
+        raise KeyError(f"NMF boundary type '{kind}' has no BC: add it to bc_map")
+    return entry if isinstance(entry, BC) else entry()
 
 
 def _cut_planes(nmf):
+    """Per block and axis, the set of interior node planes where sub-blocks must meet."""
     nb = len(nmf.dims)
-    cuts = [[set(), set(), set()] for _ in range(nb)]
+    cuts = [[set() for _ in range(3)] for _ in range(nb)]
+    queue = deque()
+
+    def add(b, ax, m):
+        if 0 < m < nmf.dims[b][ax] - 1 and m not in cuts[b][ax]:
+            cuts[b][ax].add(m)
+            queue.append((b, ax, m))
+
+    links = {}                          # (block, axis) -> [(map, partner block)]
     for p in nmf.patches:
-        sides = [(p.block, p.face, p.ranges)]
-        if p.partner:
-            sides.append((p.partner["block"], p.partner["face"], p.partner["ranges"]))
-        for b, face, ranges in sides:
-            for ax, lo, hi in _tangential(face, ranges):
-                if lo > 0:
-                    cuts[b][ax].add(lo)
-                if hi < nmf.dims[b][ax] - 1:
-                    cuts[b][ax].add(hi)
-    links = [p for p in nmf.patches if p.partner]
-    grew = True
-    while grew:                                  # carry cuts across one-to-one faces
-        grew = False
-        for p in links:
-            A, B = p.block, p.partner["block"]
-            for ax_a, ra, ax_b, rb in _axis_pairs(p):
-                for sb, sax, sr, db, dax, dr in ((A, ax_a, ra, B, ax_b, rb), (B, ax_b, rb, A, ax_a, ra)):
-                    lo, hi = sorted(sr)
-                    for m in sorted(c for c in cuts[sb][sax] if lo < c < hi):
-                        m2 = _carry(m, sr, dr)
-                        if m2 not in cuts[db][dax]:
-                            cuts[db][dax].add(m2)
-                            grew = True
+        sides = [_Side(p.block, p.face, p.ranges)]
+        if p.partner is not None:
+            q = p.partner
+            sides.append(_Side(q["block"], q["face"], q["ranges"]))
+            for mp in _maps_of(p):
+                links.setdefault((p.block, mp.ax_a), []).append((mp, q["block"]))
+                links.setdefault((q["block"], mp.ax_b), []).append((mp.inverse(), p.block))
+        for s in sides:
+            for ax, (lo, hi) in s.span.items():
+                add(s.block, ax, lo)
+                add(s.block, ax, hi)
+    while queue:
+        b, ax, m = queue.popleft()
+        for mp, other in links.get((b, ax), ()):
+            if mp.lo < m < mp.hi:
+                add(other, mp.ax_b, mp(m))
     return cuts
 
 
 def build_mesh(nodes, nmf, bc_map=None, periodic=(), ng=3, finalize=True):
-    """A `Mesh` from Plot3D node arrays and their NMF, split into sub-blocks so that every
-    sub-block face is exactly one patch.
+    """Assemble the `Mesh` of a Plot3D grid and its NMF (reference `wmles/nmf.py:206-332`).
 
     bc_map    {nmf type: BC instance or zero-argument factory}, over DEFAULT_BC_MAP
-    periodic  (nmf type, axis) pairs: such patches on the faces normal to `axis` become a
-              periodic connection between each sub-block's two faces (translation from
-              the nodes)
+    periodic  (nmf type, axis) pairs: such patches on faces normal to `axis` become a
+              periodic connection between a sub-block's two faces (translation from the
+              nodes)
     Each block gets `.origin = dict(block=<0-based NMF block>, ranges=((i0, i1), (j0, j1),
-    (k0, k1)))`, the node range it covers in its NMF block."""
-    bc_map = dict(bc_map or {})
-    periodic = {(_norm_type(t), ax) for t, ax in periodic}
+    (k0, k1)))`, the node ranges it takes from the grid block."""
+    bc_map = {_norm(k): v for k, v in (bc_map or {}).items()}
+    periodic = {(_norm(t), int(ax)) for t, ax in periodic}
     nb = len(nodes)
+    if len(nmf.dims) != nb:
+        raise ValueError(f"the grid has {nb} blocks, the NMF {len(nmf.dims)}")
     for b in range(nb):
         if tuple(nodes[b].shape[1:]) != tuple(nmf.dims[b]):
-            raise ValueError(f"block {b + 1}: grid has {nodes[b].shape[1:]} nodes, NMF says {nmf.dims[b]}")
+            raise ValueError(f"block {b + 1}: grid nodes {tuple(nodes[b].shape[1:])} against NMF {nmf.dims[b]}")
+
+    # sub-blocks, in grid-block order, then C order of their segment indices
     cuts = _cut_planes(nmf)
-    # sub-blocks, in block order then C order of their segment indices
-    edges = [[[0] + sorted(cuts[b][ax]) + [nmf.dims[b][ax] - 1] for ax in range(3)] for b in range(nb)]
-    subs, order = {}, []
+    edges = [[[0, *sorted(cuts[b][ax]), nmf.dims[b][ax] - 1] for ax in range(3)] for b in range(nb)]
+    by_index, by_ranges, order = {}, {}, []
     for b in range(nb):
-        nseg = [len(edges[b][ax]) - 1 for ax in range(3)]
-        for seg in product(*(range(n) for n in nseg)):
-            rng = tuple((edges[b][ax][seg[ax]], edges[b][ax][seg[ax] + 1]) for ax in range(3))
-            cells = tuple(r1 - r0 for r0, r1 in rng)
-            if min(cells) < ng:
-                raise ValueError(f"block {b + 1}{list(seg)} has only {cells} cells; every direction needs "
-                                 f">= ng={ng} (a 1-cell-thick span must be widened first: see respan())")
+        nseg = [len(e) - 1 for e in edges[b]]
+        for idx in product(*map(range, nseg)):
+            rng = tuple((edges[b][ax][idx[ax]], edges[b][ax][idx[ax] + 1]) for ax in range(3))
+            ncell = tuple(hi - lo for lo, hi in rng)
+            if min(ncell) < ng:
+                raise ValueError(f"block {b + 1}, sub-block {idx}: {ncell} cells, fewer than ng = {ng} "
+                                 "in some direction (a one-cell span needs respan() first)")
             X = nodes[b][:, rng[0][0]:rng[0][1] + 1, rng[1][0]:rng[1][1] + 1, rng[2][0]:rng[2][1] + 1]
-            name = f"b{b + 1}" if max(nseg) == 1 else f"b{b + 1}_" + "".join(str(s) for s in seg)
+            name = f"b{b + 1}" if nseg == [1, 1, 1] else f"b{b + 1}_" + "".join(str(i) for i in idx)
             blk = Block(name, X, ng=ng)
             blk.origin = dict(block=b, ranges=rng)
-            subs[(b, seg)] = blk
-            order.append((b, seg))
-    mesh = Mesh([subs[k] for k in order])
+            by_index[(b, idx)] = blk
+            by_ranges[(b, rng)] = blk
+            order.append(blk)
+    mesh = Mesh(order)
 
-    def on_face(key, ax, side):
-        b = key[0]
-        r = subs[key].origin["ranges"][ax]
-        return r[side] == (0 if side == 0 else nmf.dims[b][ax] - 1)
-
-    # cut planes -> interfaces between neighbouring sub-blocks of one block
-    for (b, seg), blk in subs.items():
+    # planes cut inside a grid block
+    for (b, idx), blk in by_index.items():
         for ax in range(3):
-            nxt = seg[:ax] + (seg[ax] + 1,) + seg[ax + 1:]
-            if (b, nxt) in subs:
-                mesh.connect(blk, FACE_NAME[(ax, 1)], subs[(b, nxt)], FACE_NAME[(ax, 0)])
-    # patches -> boundary conditions, one-to-one interfaces, periodic marks
-    marks = set()
+            up = tuple(i + (a == ax) for a, i in enumerate(idx))
+            if (b, up) in by_index:
+                mesh.connect(blk, FACE_NAME[(ax, 1)], by_index[(b, up)], FACE_NAME[(ax, 0)])
+
+    # every patch side, with the index maps towards its partner side
+    sides = []
     for p in nmf.patches:
-        ax, side, _ = FACES[p.face]
-        tang = _tangential(p.face, p.ranges)
-        for key, blk in subs.items():
-            if key[0] != p.block or not on_face(key, ax, side):
-                continue
-            rng = blk.origin["ranges"]
-            if not all(lo <= rng[a][0] and rng[a][1] <= hi for a, lo, hi in tang):
-                continue
-            fname = FACE_NAME[(ax, side)]
-            if p.partner is None:
-                if (p.type, ax) in periodic:
-                    marks.add((key, ax, side))
-                else:
-                    blk.set_bc(fname, _bc_for(bc_map, p.type))
-                continue
-            if (ax, side) in blk.faces:          # joined already from the partner's entry
-                continue
-            q = p.partner
-            bax, bside, _ = FACES[q["face"]]
-            want = {}
-            for ax_a, ra, ax_b, rb in _axis_pairs(p):
-                lo, hi = rng[ax_a]
-                want[ax_b] = tuple(sorted((_carry(lo, ra, rb), _carry(hi, ra, rb))))
-            hits = [k for k, sb in subs.items() if k[0] == q["block"] and on_face(k, bax, bside)
-                    and all(sb.origin["ranges"][a] == r for a, r in want.items())]
-            if len(hits) != 1:
-                raise ValueError(f"block {p.block + 1} face {p.face}: partner sub-face not found ({len(hits)})")
-            mesh.connect(blk, fname, subs[hits[0]], FACE_NAME[(bax, bside)])
-    # periodic pairs: both faces of one sub-block
-    done = set()
-    for key, ax, side in sorted(marks):
-        if (key, ax) in done:
+        if p.partner is None:
+            sides.append((_Side(p.block, p.face, p.ranges), p, None, None))
             continue
-        if (key, ax, 1 - side) not in marks:
-            raise ValueError(f"{subs[key].name}: periodic axis {ax} needs the periodic type on both faces")
-        blk = subs[key]
-        d = (np.take(blk.x_int, -1, axis=1 + ax) - np.take(blk.x_int, 0, axis=1 + ax)).reshape(3, -1)
-        if not np.allclose(d, d[:, :1], atol=1e-9 * max(np.abs(blk.x_int).max(), 1.0)):
-            raise ValueError(f"{blk.name}: axis {ax} faces are not related by a pure translation")
-        mesh.connect(blk, FACE_NAME[(ax, 0)], blk, FACE_NAME[(ax, 1)], shift=tuple(-d[:, 0]))
-        done.add((key, ax))
+        q = p.partner
+        maps = _maps_of(p)
+        sides.append((_Side(p.block, p.face, p.ranges), p, q, maps))
+        sides.append((_Side(q["block"], q["face"], q["ranges"]), p,
+                      dict(block=p.block, face=p.face), [m.inverse() for m in maps]))
+
+    periodic_faces = set()
+    for blk in order:
+        b, rng = blk.origin["block"], blk.origin["ranges"]
+        for ax, side in product(range(3), range(2)):
+            if rng[ax][side] != (0 if side == 0 else nmf.dims[b][ax] - 1):
+                continue                                  # an inner plane: connected above
+            owners = [s for s in sides if s[0].block == b and (s[0].axis, s[0].side) == (ax, side)
+                      and s[0].covers(rng)]
+            if not owners:
+                continue                                  # left open: finalize() reports it
+            if len(owners) > 1:
+                raise ValueError(f"{blk.name}.{FACE_NAME[(ax, side)]} lies in {len(owners)} NMF patches")
+            own, p, other, maps = owners[0]
+            face = FACE_NAME[(ax, side)]
+            if other is None:
+                if (p.type, ax) in periodic:
+                    periodic_faces.add((blk.name, ax, side))
+                else:
+                    blk.set_bc(face, _bc_instance(bc_map, p.type))
+                continue
+            if (ax, side) in blk.faces:
+                continue                                  # joined from the partner's side
+            pax, pside, _ = FACES[other["face"]]
+            pb = other["block"]
+            want = [None, None, None]
+            for mp in maps:
+                lo, hi = rng[mp.ax_a]
+                want[mp.ax_b] = tuple(sorted((mp(lo), mp(hi))))
+            pe = edges[pb][pax]
+            want[pax] = (pe[0], pe[1]) if pside == 0 else (pe[-2], pe[-1])
+            partner = by_ranges.get((pb, tuple(want)))
+            if partner is None:
+                raise ValueError(f"{blk.name}.{face}: no sub-block of grid block {pb + 1} has the "
+                                 f"face its one-to-one patch maps to (node ranges {tuple(want)})")
+            mesh.connect(blk, face, partner, FACE_NAME[(pax, pside)])
+
+    # periodic translations between the two faces of one sub-block
+    for blk in order:
+        for ax in range(3):
+            marks = [(blk.name, ax, s) in periodic_faces for s in range(2)]
+            if not any(marks):
+                continue
+            if not all(marks):
+                raise ValueError(f"{blk.name}: periodic type on only one {'ijk'[ax]} face")
+            d = (np.take(blk.x_int, blk.n[ax], axis=1 + ax) - np.take(blk.x_int, 0, axis=1 + ax)).reshape(3, -1)
+            if np.abs(d - d[:, :1]).max() > 1e-9 * max(float(np.abs(blk.x_int).max()), 1.0):
+                raise ValueError(f"{blk.name}: its two {'ijk'[ax]} faces are not a translation apart")
+            mesh.connect(blk, FACE_NAME[(ax, 0)], blk, FACE_NAME[(ax, 1)], shift=tuple(-d[:, 0]))
+
     if finalize:
         mesh.finalize()
     return mesh
 
 
 def describe(mesh):
-    """Text summary: blocks, sizes, origin, the BC or partner of every face, total cells."""
-    lines, total = [], 0
-    for i, b in enumerate(mesh.blocks):
-        total += int(np.prod(b.n))
-        o = getattr(b, "origin", None)
-        src = f"  (block {o['block'] + 1}, nodes {[tuple(r) for r in o['ranges']]})" if o else ""
-        lines.append(f"[{i}] {b.name}: cells {b.n}{src}")
-        for (ax, s), f in sorted(b.faces.items()):
+    """Text summary of an assembled mesh: each block's cells and origin, and what each face
+    is joined to (an interface with its shift, or the BC class)."""
+    out, total = [], 0
+    for i, blk in enumerate(mesh.blocks):
+        total += int(np.prod(blk.n))
+        org = getattr(blk, "origin", None)
+        where = "" if org is None else f" from grid block {org['block'] + 1} nodes {list(org['ranges'])}"
+        out.append(f"{i:3d} {blk.name} {blk.n[0]}x{blk.n[1]}x{blk.n[2]} cells{where}")
+        for key in sorted(blk.faces):
+            f = blk.faces[key]
             if isinstance(f, Interface):
-                other = mesh.blocks[f.nb]
-                sh = "" if not np.any(f.shift) else f"  shift={[float(v) + 0.0 for v in np.round(f.shift, 6)]}"
-                lines.append(f"      {FACE_NAME[(ax, s)]}: -> {other.name}.{FACE_NAME[tuple(f.src_face)]}{sh}")
+                tgt = f"{mesh.blocks[f.nb].name}.{FACE_NAME[tuple(f.src_face)]}"
+                shift = "" if not np.any(f.shift) else " shift " + " ".join(f"{v:.6g}" for v in f.shift)
+                out.append(f"      {FACE_NAME[key]} = {tgt}{shift}")
             else:
-                lines.append(f"      {FACE_NAME[(ax, s)]}: {type(f).__name__}")
-    lines.append(f"total cells: {total}")
-    return "\n".join(lines)
+                out.append(f"      {FACE_NAME[key]} : {type(f).__name__}")
+    out.append(f"cells: {total}")
+    return "\n".join(out)
