@@ -233,6 +233,18 @@ int wmx_plan_tiling(const wmx_plan* plan, int32_t* ci_len, int32_t* order, int32
 /* CFL time step (per case when the plan is an ensemble) into d_dt[n_cases].
    Replaces Solver.max_dt / _dt (wmles/solver.py:135-154). */
 int wmx_max_dt(wmx_plan* plan, double* const* q, double* d_dt, void* stream);
+/* Allocation-free CFL time step (no counterpart in the reference: it is for callers that
+   cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  The same result as wmx_max_dt, bit for bit, with the
+   per-workgroup CFL partials in a static device array of the library (one per GPU,
+   WMX_NOALLOC_MAX_PARTIALS doubles = 512 KiB, held until the process ends) instead of
+   the plan's workspace; the plan needs no workspace bound.  Makes no hipMalloc / hipFree.
+   Plans whose interior cells need more partials than the array holds (more than
+   WMX_NOALLOC_MAX_PARTIALS x 1024 interior cells, about 67 M) are rejected with an error.
+   Every call on one GPU shares that GPU's array: calls that overlap on one GPU (other
+   streams, other threads) race and must be serialised by the caller. */
+#define WMX_NOALLOC_MAX_PARTIALS 65536
+int wmx_max_dt_noalloc(wmx_plan* plan, double* const* q, double* d_dt, void* stream);
 /* Constant-mass-flux constraint in place with the plan's per-case targets.
    Replaces ConstantMassFlux.__call__ (wmles/solver.py:22-36). */
 int wmx_mass_flux(wmx_plan* plan, double* const* q, void* stream);

@@ -95,6 +95,7 @@ def _load():
         "wmx_rhs": ([P, PP, PP, C.c_double, P], C.c_int),
         "wmx_advance": ([P, PP, PP, C.c_double, P, C.c_double, P], C.c_int),
         "wmx_max_dt": ([P, PP, P, P], C.c_int),
+        "wmx_max_dt_noalloc": ([P, PP, P, P], C.c_int),
         "wmx_step": ([P, PP, PP, P, C.c_double, C.c_int32, P], C.c_int),
         "wmx_plan_set_tiling": ([P, C.c_int32, C.c_int32], C.c_int),
         "wmx_tile_model": ([C.c_int32, P, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -756,6 +757,12 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     wmx_plan_destroy(p);
     return fail("hipMalloc of block descriptors failed");
   }
+  // the descriptors without workspace pointers, so that calls which read only the state
+  // and the geometry (wmx_max_dt_noalloc) work before a workspace is bound
+  if (hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * nblocks, hipMemcpyHostToDevice) != hipSuccess) {
+    wmx_plan_destroy(p);
+    return fail("upload of block descriptors failed");
+  }
   *out = p;
   return 0;
 }
@@ -1066,6 +1073,58 @@ int wmx_max_dt(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
                             p->d_part, nullptr, d_dt, s);
     return e;
   }));
+  return 0;
+}
+
+// ---- allocation-free CFL dt -------------------------------------------------------
+// The CFL partials of wmx_max_dt_noalloc live in this file-scope device array (one copy
+// per GPU, held until the process ends).  Host code never passes the array's name to a
+// kernel: its device address is looked up once per device (hipGetSymbolAddress, with the
+// device current) and kept in g_scratch_ptr.  Every partial is written by the partial
+// launch before the final launch reads it, so nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[WMX_NOALLOC_MAX_PARTIALS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_of_current_device(double** out) {
+  int dev = -1;
+  WMX_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail("device index out of range for the static scratch");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    WMX_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail("hipGetSymbolAddress returned no address for the static scratch");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+int wmx_max_dt_noalloc(wmx_plan* p, double* const* q, double* d_dt, void* stream) {
+  if (!p) return fail("null plan");
+  if (!q || !d_dt) return fail("null argument");
+  for (int b = 0; b < p->nb; ++b)
+    if (!q[b]) return fail("null state pointer");
+  const int npart = nwg_of(p->pre_int);
+  if (npart > WMX_NOALLOC_MAX_PARTIALS)
+    return fail("wmx_max_dt_noalloc: the plan needs " + std::to_string(npart) +
+                " CFL partials, more than the static scratch holds (" +
+                std::to_string(WMX_NOALLOC_MAX_PARTIALS) + ", i.e. about " +
+                std::to_string((long long)WMX_NOALLOC_MAX_PARTIALS * 256 * INT_CELLS) +
+                " interior cells); use wmx_max_dt");
+  double* part = nullptr;
+  if (scratch_of_current_device(&part)) return 1;
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q);
+  // the same two launches as wmx_max_dt, with the partials in the static scratch
+  WMX_CHECK(launch_dt_partial(p->d_blks, p->d_pre_int, p->nb, npart, pq, p->nd, part, s));
+  WMX_CHECK(launch_case_final(0, p->n_cases, p->case_wg_ident ? nullptr : p->d_case_wg, p->d_case_off,
+                              part, nullptr, d_dt, s));
   return 0;
 }
 
