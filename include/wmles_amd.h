@@ -41,7 +41,10 @@ enum {
   WMX_BC_FARFIELD = 4,       /* CharacteristicFarfield    bc.py:80-115  */
   WMX_BC_EXTRAPOLATE = 5,    /* Extrapolate               bc.py:118-125 */
   WMX_BC_WALLMODEL = 6,      /* WallModel (slip fill + flux override) bc.py:128-166 */
-  WMX_BC_INFLOW = 7          /* SmoothRampInflow          smooth_ramp.py:116-197 */
+  WMX_BC_INFLOW = 7,         /* SmoothRampInflow          smooth_ramp.py:116-197 */
+  WMX_BC_EXTERNAL = 8        /* a BC the caller fills itself (a user fill(q, blk, geom, gas,
+                                axis, side) without a HIP descriptor, bc.py:1-8): the sweep
+                                gives it a phase of its own and leaves it to the caller */
 };
 
 /* inviscid schemes: reference wmles/schemes.py */
@@ -197,8 +200,23 @@ int wmx_compute_geometry(const double* X, const int32_t n_ghosted[3], double* vo
                          double* xf0, double* xf1, double* xf2, void* stream);
 
 /* Ghost sweep in place (interior untouched).  Replaces Solver.apply_bcs
-   (wmles/solver.py:59-73). */
+   (wmles/solver.py:59-73).  Plans with WMX_BC_EXTERNAL faces refuse it (and wmx_rhs,
+   wmx_advance, wmx_step): their caller runs the sweep phase by phase. */
 int wmx_apply_bcs(wmx_plan* plan, double* const* q, double time, void* stream);
+/* The sweep's phases in execution order: records[4 * i] = 1 when phase i is the fill of
+   a WMX_BC_EXTERNAL face (then records[4 i + 1..3] = block, axis, side), else 0 (and -1).
+   records may be NULL to get the count.  With the reference's sequential order
+   (solver.py:59-73) a caller fills the external faces itself between
+   wmx_apply_bcs_phases calls. */
+int wmx_plan_sweep_phases(const wmx_plan* plan, int32_t* nphases, int32_t* records);
+/* Runs phases [first, last) of the sweep in place; the range must hold no external
+   phase. */
+int wmx_apply_bcs_phases(wmx_plan* plan, double* const* q, int32_t first, int32_t last, double time,
+                         void* stream);
+/* R = RHS(q) with zero ghosts, for a q whose ghosts the caller has filled (no sweep; q is
+   not modified).  The RHS part of Solver._rhs after apply_bcs (wmles/solver.py:97-106);
+   the callers' own sources are added by the caller. */
+int wmx_rhs_filled(wmx_plan* plan, double* const* q, double* const* r, void* stream);
 /* R = RHS(q) with zero ghosts; q's ghost cells are overwritten by the sweep.
    Replaces Solver.rhs / _rhs (wmles/solver.py:97-106,159-160). */
 int wmx_rhs(wmx_plan* plan, double* const* q, double* const* r, double time, void* stream);
@@ -224,6 +242,18 @@ int wmx_step(wmx_plan* plan, double* const* q0, double* const* qout, double* d_d
    count, the default); order 0 (block/i/j/k) or 1 (contiguous spans interleaved
    across the 8 XCDs, the default). */
 int wmx_plan_set_tiling(wmx_plan* plan, int32_t ci_len, int32_t order);
+/* Persistent stages (the batching form's launch structure; no reference counterpart):
+   on != 0 runs every RK stage of wmx_advance / wmx_step as ONE cooperative launch -- a
+   grid of the kernel's co-resident maximum walks the stage's ghost-sweep phases, the
+   primitive records, the gradients (with the per-case dt min and the Newton wall faces)
+   and the flux tiles, with grid-wide barriers between them -- instead of one launch per
+   kernel class; per step the ghost-shell save / restore and the mass-flux constraint stay
+   separate launches.  Same work and same results (bit for bit) as the separate launches.
+   FP32-bulk plans without time-dependent inflow, sponge sources, the learned wall model or
+   caller-filled faces only (others are refused with an error); not captured into graphs
+   (wmx_step's use_graph is ignored while it is on).  on == 0 restores the separate
+   launches. */
+int wmx_plan_set_persistent(wmx_plan* plan, int32_t on);
 /* The automatic choice without a plan (host only): i-planes per tile and tile count
    for blocks of interior sizes dims[3 * nblocks] in the given form on cu_count CUs. */
 int wmx_tile_model(int32_t nblocks, const int32_t* dims, int32_t precision, int32_t cu_count,
@@ -268,6 +298,29 @@ int wmx_fill_face(const wmx_block_desc* block, int32_t axis, int32_t side,
 int wmx_wall_model_eval(const wmx_numerics_desc* numerics, int32_t mode, int64_t n,
                         const double* a, const double* b, const double* c, const double* d,
                         double T_wall, double* out0, double* out1, void* stream);
+
+/* ---- the reference's operator callables, evaluated in FP64 by the step's own device code ---- */
+/* Inviscid flux incl. area of the numerics' scheme (KEP / KEEP of order 2 or 4,
+   MUSCL-HLLC, Hybrid) through the n_d + 1 interior faces of direction d over the
+   interior tangential cells: F (5, n0 + (d==0), n1 + (d==1), n2 + (d==2)) C order.
+   w: (5, N0, N1, N2) ghosted primitives [rho, u, v, w, p] (ng = 3); S_d: the block's
+   face vectors of direction d (3, .. N_d + 1 ..); theta: (N0, N1, N2) Ducros sensor
+   (Hybrid only, else may be NULL).  Replaces scheme(blk, geom, w, gas, d, theta)
+   (reference wmles/schemes.py:3-5, KEP 57-70, KEEP 72-98, MUSCLHLLC 140-152, Hybrid 155-166). */
+int wmx_scheme_flux(const wmx_numerics_desc* numerics, const int32_t n[3], int32_t d, const double* w,
+                    const double* S_d, const double* theta, double* F, void* stream);
+/* Eddy viscosity mu_t = rho nu_t(g, delta) of the numerics' SGS model on n points:
+   g (3, 3, n) with g[i][j] = du_i/dx_j, delta and rho (n).  Replaces the callable of
+   make_sgs(name, **kw)(g, delta, rho) (reference wmles/sgs.py:6-50). */
+int wmx_sgs_eval(const wmx_numerics_desc* numerics, int64_t n, const double* g, const double* delta,
+                 const double* rho, double* mu_t, void* stream);
+/* Wall-model flux of WallModel face (axis, side) of one block with the equilibrium model:
+   F (5, n_t1, n_t2) incl. area and the sign of the face's outward side, tau_w and q_w
+   (n_t1, n_t2).  w (5, N...) ghosted primitives and T (N...).  The block descriptor
+   supplies n, the face's T_wall / n_match and the geometry (S, xf, xc).  Replaces
+   WallModel.wall_flux(w, T, blk, geom, gas, axis, side) (reference wmles/bc.py:145-166). */
+int wmx_wall_flux(const wmx_block_desc* block, int32_t axis, int32_t side, const wmx_numerics_desc* numerics,
+                  const double* w, const double* T, double* F, double* tau_w, double* q_w, void* stream);
 
 /* ---- learned wall model (no counterpart in the reference; see wmx_wall_net_desc) ---- */
 /* Parameter counts: FP32 master, BF16 inference copy, FP32 inference copy. */

@@ -281,6 +281,8 @@ def make_scheme(ns, sc):
 
 def make_bc(ns, bc, props=None, arrays=None):
     kind = bc["kind"]
+    if kind == "Object":                      # a BC object built by the test itself
+        return bc["obj"]
     if kind == "WallModel":
         wm = bc["model"] if "model" in bc else _resolve(ns, "EquilibriumWallModel")(**bc.get("wm", {}))
         return _resolve(ns, "WallModel")(wm, T_wall=bc.get("T_wall"), n_match=bc.get("n_match", 1))

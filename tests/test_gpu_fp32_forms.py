@@ -153,13 +153,18 @@ def test_fp32_ramp_rhs_close_to_reference_golden():
 
 def test_fp32_ramp_steps_close_to_fp64():
     """Smooth ramp 48 x 24 x 16 (inflow + sponge + wall model): FP32-bulk against FP64
-    for one RHS, then after 20 steps the spanwise-mean u(x, y) and the wall-mean tau_w."""
+    for one RHS of a perturbed state, then after 20 steps from the smooth initial state
+    the spanwise-mean u(x, y) and the wall-mean tau_w.  (The RHS of the smooth initial
+    state itself is a near-cancellation of O(rho U / dx) face fluxes -- the flow is nearly
+    steady there -- so its max-normalised difference measures the FP32 face-flux rounding
+    against a vanishing RHS, not the form's accuracy; the perturbed state has the RHS
+    magnitude of a developed flow.)"""
     spec, arrays, prim = cases.ramp_spec(wmx, 48, 24, 16, n_modes=16)
     out = {}
     for prec in ("fp64", "fp32"):
         mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": prec})
         Q0 = s.init_state(prim)
-        R = host(s.rhs(Q0, 0.0))
+        R = host(s.rhs(cases.interior_noise(mesh.blocks, host(Q0), 7, 0.02), 0.0))
         Q, t = Q0, 0.0
         for _ in range(20):
             Qn, dt = s.step(Q, time=t)

@@ -148,5 +148,5 @@ def test_restart_round_trips_through_a_solver(tmp_path):
         f.write('<?xml version="1.0"?>\n<VTKFile type="Collection" version="0.1"><Collection>'
                 f'<DataSet timestep="{t + 1.0!r}" part="0" file="chan_block0_000005.vtu"/>'
                 '</Collection></VTKFile>')
-    with pytest.raises(ValueError, match="block indices"):
+    with pytest.raises(ValueError, match="holds blocks"):
         IO.read_restart(one, s)

@@ -17,7 +17,7 @@ LIB_PATH = os.environ.get("WMX_LIB", os.path.join(_HERE, "libwmles_amd.so"))
 
 WMX_MAX_BLOCKS = 64
 FACE_INTERFACE, BC_SLIP, BC_NOSLIP, BC_DIRICHLET, BC_FARFIELD, BC_EXTRAPOLATE, BC_WALLMODEL, \
-    BC_INFLOW = range(8)
+    BC_INFLOW, BC_EXTERNAL = range(9)
 SCHEME_KEP, SCHEME_KEEP, SCHEME_MUSCLHLLC, SCHEME_HYBRID = range(4)
 LIMITER_ALBADA, LIMITER_MINMOD = range(2)
 SGS_NONE, SGS_SMAGORINSKY, SGS_WALE, SGS_VREMAN = range(4)
@@ -96,8 +96,16 @@ def _load():
         "wmx_advance": ([P, PP, PP, C.c_double, P, C.c_double, P], C.c_int),
         "wmx_max_dt": ([P, PP, P, P], C.c_int),
         "wmx_max_dt_noalloc": ([P, PP, P, P], C.c_int),
+        "wmx_plan_sweep_phases": ([P, C.POINTER(C.c_int32), P], C.c_int),
+        "wmx_apply_bcs_phases": ([P, PP, C.c_int32, C.c_int32, C.c_double, P], C.c_int),
+        "wmx_rhs_filled": ([P, PP, PP, P], C.c_int),
+        "wmx_scheme_flux": ([C.POINTER(NumericsDesc), C.POINTER(C.c_int32), C.c_int32, P, P, P, P, P], C.c_int),
+        "wmx_sgs_eval": ([C.POINTER(NumericsDesc), C.c_int64, P, P, P, P, P], C.c_int),
+        "wmx_wall_flux": ([C.POINTER(BlockDesc), C.c_int32, C.c_int32, C.POINTER(NumericsDesc), P, P, P, P, P, P],
+                          C.c_int),
         "wmx_step": ([P, PP, PP, P, C.c_double, C.c_int32, P], C.c_int),
         "wmx_plan_set_tiling": ([P, C.c_int32, C.c_int32], C.c_int),
+        "wmx_plan_set_persistent": ([P, C.c_int32], C.c_int),
         "wmx_tile_model": ([C.c_int32, P, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "wmx_plan_tiling": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "wmx_mass_flux": ([P, PP, P], C.c_int),

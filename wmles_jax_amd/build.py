@@ -6,7 +6,9 @@ Every translation unit is compiled with packed FP32 disabled.  The ghost-fill /
 geometry and reduction units are compiled without floating-point contraction so
 that their results round exactly like the reference's float64 host arithmetic; the
 RHS unit contracts multiply-adds into FMAs.  The FP64 and Hybrid FP32-bulk flux kernels
-(flux_ilp.hip) are scheduled by LLVM's max-ILP machine scheduler.
+(flux_ilp.hip) are scheduled by LLVM's max-ILP machine scheduler.  The persistent stage
+kernel (stage.hip) and its bodies (stage_off.hip, stage_fast.hip) are relocatable device
+code, device-linked into the same library.
 """
 import os
 import subprocess
@@ -32,7 +34,13 @@ UNITS = {
     "flux.hip": ["-ffp-contract=fast"],
     "flux_ilp.hip": ["-ffp-contract=fast", "-mllvm", "-amdgpu-sched-strategy=max-ilp"],
     "mlp.hip": ["-ffp-contract=fast"],
+    "ops.hip": ["-ffp-contract=fast"],
     "capi.hip": [],
+    # the persistent stage kernel and its bodies: relocatable device code, each body unit
+    # with the FP contraction of the unit whose kernels it shares code with
+    "stage_off.hip": ["-ffp-contract=off", "-fgpu-rdc"],
+    "stage_fast.hip": ["-ffp-contract=fast", "-fgpu-rdc"],
+    "stage.hip": ["-fgpu-rdc"],
 }
 
 
@@ -51,7 +59,10 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(6, len(UNITS))) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), UNITS.items()))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs]
+    # -fgpu-rdc --hip-link: the relocatable units are device-linked (with LTO; each
+    # instruction keeps its unit's contraction) beside the self-contained ones
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fgpu-rdc", "--hip-link", "-shared", "-fPIC", "-o", tmp,
+           *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

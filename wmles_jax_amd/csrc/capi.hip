@@ -75,6 +75,8 @@ struct wmx_plan {
   std::vector<int> phase_max;
   GhostOp* d_ops = nullptr;
   std::vector<std::vector<int>> phase_inflow;  // block ids of inflow fills per phase
+  std::vector<GhostOp> phase_ext;     // per phase: the external fill it stands for (blk -1: none)
+  bool has_external = false;          // some face is WMX_BC_EXTERNAL: the caller runs the sweep
   bool fuse_last = false;   // the last phase is k-face fills only: a step's primitive launches
                             // evaluate those ghosts (BlockDev::zfuse) and skip its fill launch
   // sponge row means in head workgroups of the primitive launch (every block's rows of
@@ -125,6 +127,15 @@ struct wmx_plan {
   char* ws = nullptr;
   double* d_part = nullptr;
   double* d_delta = nullptr;
+  // persistent stage launches (FP32-bulk plans without inflow, sponge, learned wall model
+  // or caller-filled faces): one cooperative launch per RK stage
+  bool persistent = false;
+  bool stage_dirty = true;            // d_stage needs re-uploading (tiling changed)
+  StageStatic* d_stage = nullptr;
+  Ptrs* d_pbuf = nullptr;
+  int* d_fill_off = nullptr;
+  int* d_fill_max = nullptr;
+  int stage_grid = 0;                 // persistent grid in use (0: not yet launched)
   // kernel-class event timing
   int prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -185,6 +196,8 @@ void op_boxes(const wmx_plan& p, int b, int a, int s, std::vector<Box>* rd, Box*
     rd->push_back(r);
   } else if (f.kind == WMX_BC_DIRICHLET || f.kind == WMX_BC_INFLOW) {
     // reads no state
+  } else if (f.kind == WMX_BC_EXTERNAL) {
+    rd->push_back(full_box(b, B.N));      // a caller's fill may read anything in its block
   } else {
     Box r = full_box(b, B.N);
     if (f.kind == WMX_BC_EXTRAPOLATE) {
@@ -282,8 +295,10 @@ int build_phases(wmx_plan& p, int passes) {
   p.phase_off.assign(1, 0);
   p.phase_max.clear();
   p.phase_inflow.clear();
+  p.phase_ext.clear();
   auto close = [&]() {
     if (cur.empty()) return;
+    p.phase_ext.push_back(GhostOp{-1, -1, -1, 0});
     int mx = 0;
     std::vector<int> inflow;
     for (auto& c : cur) {
@@ -306,6 +321,15 @@ int build_phases(wmx_plan& p, int passes) {
           int t1, t2;
           tangents(a, &t1, &t2);
           x.op = GhostOp{b, a, s, B.ng * B.N[t1] * B.N[t2]};
+          if (B.faces[2 * a + s].kind == WMX_BC_EXTERNAL) {
+            // a phase of its own, with no op for the fill kernel
+            close();
+            p.phase_off.push_back((int)p.ops.size());
+            p.phase_max.push_back(0);
+            p.phase_inflow.push_back({});
+            p.phase_ext.push_back(x.op);
+            continue;
+          }
           op_boxes(p, b, a, s, &x.rd, &x.wr);
           bool conflict = false;
           for (auto& c : cur) {
@@ -415,6 +439,7 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
              bool zfuse = false) {
   const int nph = (int)p->phase_max.size() - (zfuse && p->fuse_last ? 1 : 0);
   for (int ph = 0; ph < nph; ++ph) {
+    if (p->phase_ext[ph].blk >= 0) continue;          // the caller's fill
     const int o = p->phase_off[ph], n = p->phase_off[ph + 1] - o;
     WMX_CHECK(timed(p, K_FILL, s, [&] { return launch_fill_cells(p->d_blks, p->d_ops + o, n, p->phase_max[ph], q, p->nd.gas, s); }));
     for (int blk : p->phase_inflow[ph]) {
@@ -434,9 +459,9 @@ int run_fill(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, 
 // interior state and the per-case minimum goes to dt_out before the gradients
 // zfuse: a step's stage (the sweep's last phase evaluated by the primitive launch)
 int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef, hipStream_t s,
-              double* dt_out = nullptr, bool zfuse = false) {
-  zfuse = zfuse && p->fuse_last;
-  if (run_fill(p, q, time, d_dt, coef, s, zfuse)) return 1;
+              double* dt_out = nullptr, bool zfuse = false, bool fill = true) {
+  zfuse = zfuse && p->fuse_last && fill;
+  if (fill && run_fill(p, q, time, d_dt, coef, s, zfuse)) return 1;
   SpongeHead sh;
   if (p->d_sponge_pre) {
     sh.pre = p->d_sponge_pre;
@@ -602,7 +627,9 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
         if (p->hb[b].N[2] > 1024) { delete p; return fail("inflow fill supports N_k <= 1024"); }
       } else if (fc.kind == WMX_BC_WALLMODEL) {
         if (fc.n_match < 1 || fc.n_match > p->hb[b].n[a]) { delete p; return fail("n_match out of range"); }
-      } else if (fc.kind < 0 || fc.kind > WMX_BC_INFLOW) {
+      } else if (fc.kind == WMX_BC_EXTERNAL) {
+        p->has_external = true;
+      } else if (fc.kind < 0 || fc.kind > WMX_BC_EXTERNAL) {
         delete p;
         return fail("unknown face kind");
       }
@@ -616,7 +643,8 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     p->inflows.push_back(InflowDev{in.n_tab, in.n_modes, 0, in.table, in.modes, in.U_inf, in.L,
                                    in.rho_inf, in.p_inf});
   }
-  p->passes = effective_passes(*p, nm->bc_passes);
+  // (what a caller's fill writes is unknown: every pass of such a plan is kept)
+  p->passes = p->has_external ? nm->bc_passes : effective_passes(*p, nm->bc_passes);
   build_phases(*p, p->passes);
   // The last phase's values are read by nothing else in a step but the primitive kernel
   // (the later kernels read the primitive records or interior states; the step's output
@@ -624,7 +652,7 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
   // primitive kernels of the step's stages evaluate them in place of its launch.
   {
     const int nph = (int)p->phase_max.size();
-    bool ok = nph > 0;
+    bool ok = nph > 0 && !p->has_external;
     if (ok)
       for (int o = p->phase_off[nph - 1]; o < p->phase_off[nph]; ++o) {
         const GhostOp& op = p->ops[o];
@@ -786,6 +814,10 @@ int wmx_plan_destroy(wmx_plan* p) {
   (void)hipFree(p->d_sponge_pre);
   (void)hipFree(p->d_case_dir);
   (void)hipFree(p->d_case_target);
+  (void)hipFree(p->d_stage);
+  (void)hipFree(p->d_pbuf);
+  (void)hipFree(p->d_fill_off);
+  (void)hipFree(p->d_fill_max);
   delete p;
   return 0;
 }
@@ -904,6 +936,7 @@ int wmx_plan_set_gauge(wmx_plan* p, double p_ref, double T_ref) {
 }
 
 int wmx_plan_set_wall_net(wmx_plan* p, const wmx_wall_net_desc* net) {
+  if (p && net && p->persistent) return fail("disable persistent stages before setting a learned wall model");
   if (!p) return fail("null plan");
   if (net) {
     if (check_net(net)) return 1;
@@ -991,13 +1024,71 @@ int wmx_compute_geometry(const double* X, const int32_t N[3], double* vol, doubl
   return 0;
 }
 
+namespace {
+int refuse_external(const wmx_plan* p) {
+  if (p->has_external)
+    return fail("the plan has faces the caller fills (WMX_BC_EXTERNAL): run the sweep with "
+                "wmx_apply_bcs_phases and the RHS with wmx_rhs_filled");
+  return 0;
+}
+}  // namespace
+
 int wmx_apply_bcs(wmx_plan* p, double* const* q, double time, void* stream) {
-  if (check_plan(p)) return 1;
+  if (check_plan(p) || refuse_external(p)) return 1;
   return run_fill(p, make_ptrs(p, q), time, nullptr, 0.0, as_stream(stream));
 }
 
-int wmx_rhs(wmx_plan* p, double* const* q, double* const* r, double time, void* stream) {
+int wmx_plan_sweep_phases(const wmx_plan* p, int32_t* nphases, int32_t* rec) {
+  if (!p || !nphases) return fail("null argument");
+  const int n = (int)p->phase_max.size();
+  *nphases = n;
+  if (rec)
+    for (int i = 0; i < n; ++i) {
+      const GhostOp& e = p->phase_ext[i];
+      rec[4 * i] = e.blk >= 0 ? 1 : 0;
+      rec[4 * i + 1] = e.blk;
+      rec[4 * i + 2] = e.axis;
+      rec[4 * i + 3] = e.side;
+    }
+  return 0;
+}
+
+int wmx_apply_bcs_phases(wmx_plan* p, double* const* q, int32_t first, int32_t last, double time,
+                         void* stream) {
   if (check_plan(p)) return 1;
+  if (!q) return fail("null argument");
+  const int n = (int)p->phase_max.size();
+  if (first < 0 || last > n || first > last) return fail("phase range out of bounds");
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q);
+  for (int ph = first; ph < last; ++ph) {
+    if (p->phase_ext[ph].blk >= 0) return fail("the phase range holds a caller-filled face");
+    const int o = p->phase_off[ph], cnt = p->phase_off[ph + 1] - o;
+    WMX_CHECK(launch_fill_cells(p->d_blks, p->d_ops + o, cnt, p->phase_max[ph], pq, p->nd.gas, s));
+    for (int blk : p->phase_inflow[ph]) {
+      const BlockDev& B = p->hb[blk];
+      WMX_CHECK(launch_inflow(p->d_blks, blk, B.N[1], B.N[2], B.ng, p->inflows[B.faces[0].inflow_id], pq,
+                              p->nd.gas, time, nullptr, 0.0, s));
+    }
+  }
+  return 0;
+}
+
+int wmx_rhs_filled(wmx_plan* p, double* const* q, double* const* r, void* stream) {
+  if (check_plan(p)) return 1;
+  if (!q || !r) return fail("null argument");
+  hipStream_t s = as_stream(stream);
+  Ptrs pq = make_ptrs(p, q), pr = make_ptrs(p, r);
+  if (run_cells(p, pq, 0.0, nullptr, 0.0, s, nullptr, false, false)) return 1;
+  WMX_CHECK(launch_flux(0, p->d_blks, p->d_tiles, (int)p->tiles.size(), p->ci_len, p->nd, pq, pq, pr,
+                        nullptr, 0.0, s));
+  WMX_CHECK(launch_shell_finish(0, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), pq, pr,
+                                nullptr, 0.0, s));
+  return 0;
+}
+
+int wmx_rhs(wmx_plan* p, double* const* q, double* const* r, double time, void* stream) {
+  if (check_plan(p) || refuse_external(p)) return 1;
   hipStream_t s = as_stream(stream);
   Ptrs pq = make_ptrs(p, q), pr = make_ptrs(p, r);
   if (run_cells(p, pq, time, nullptr, 0.0, s)) return 1;
@@ -1031,10 +1122,83 @@ namespace {
 // dt_out != nullptr (then dt_out == d_dt): the step's dt is computed here, from q0, by the
 // stage-1 primitive launch.  The fill before it changes ghosts only, and stage 1's time is
 // `time` itself (solver.py:125-129), so that fill takes no dt.
+// Why a plan cannot run persistent stages ("" if it can).
+std::string persistent_blocker(const wmx_plan* p) {
+  if (p->nd.prec != WMX_PREC_FP32_BULK) return "persistent stages are built for the FP32-bulk form";
+  for (auto& ph : p->phase_inflow)
+    if (!ph.empty()) return "plans with a time-dependent inflow run separate launches";
+  if (p->nd.n_sponge > 0) return "plans with sponge sources run separate launches";
+  if (p->nd.net.w) return "plans with the learned wall model run separate launches";
+  if (p->has_external) return "plans with caller-filled faces run separate launches";
+  return "";
+}
+
+int upload_stage(wmx_plan* p) {
+  if (!p->stage_dirty) return 0;
+  StageStatic st;
+  std::memset(&st, 0, sizeof(st));
+  st.blks = p->d_blks;
+  st.nd = p->nd;
+  st.ops = p->d_ops;
+  st.fill_off = p->d_fill_off;
+  st.fill_max = p->d_fill_max;
+  st.pre_all = p->d_pre_all;
+  st.nblk = p->nb;
+  st.nprim = nwg_of(p->pre_all);
+  st.pre_box = p->d_pre_box;
+  st.ngrad = nwg_of(p->pre_box);
+  st.wall_ops = p->d_wall;
+  st.nwall_ops = (int)p->wall.size();
+  st.wall_wgx = st.nwall_ops > 0 && p->wall_max > 0 ? (p->wall_max + 255) / 256 : 0;
+  st.dt_list = p->case_wg_all_ident ? nullptr : p->d_case_wg_all;
+  st.dt_off = p->d_case_off_all;
+  st.tiles = p->d_tiles;
+  st.ntiles = (int)p->tiles.size();
+  st.ci_len = p->ci_len;
+  st.pbuf = p->d_pbuf;
+  WMX_CHECK(hipMemcpy(p->d_stage, &st, sizeof(st), hipMemcpyHostToDevice));
+  p->stage_dirty = false;
+  return 0;
+}
+
+// One step's stages as persistent launches (same work, same bits as the separate launches
+// of advance_impl below): the ghost shell saved, three cooperative stage launches, the
+// shell restored, the mass-flux constraint.
+int advance_persistent(wmx_plan* p, double* const* q0, double* const* qout, double dt, const double* d_dt,
+                       double* dt_out, void* stream) {
+  if (p->capturing) return fail("persistent stages are not captured into graphs");
+  if (upload_stage(p)) return 1;
+  hipStream_t s = as_stream(stream);
+  Ptrs P0 = make_ptrs(p, q0), PO = make_ptrs(p, qout);
+  WMX_CHECK(launch_shell_save(p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, s));
+  const int sc = scheme_config(p->nd);
+  const int nph = (int)p->phase_max.size();
+  const int wx = !p->wall.empty() && p->wall_max > 0 ? (p->wall_max + 255) / 256 : 0;
+  for (int mode = 1; mode <= 3; ++mode) {
+    StageCall c;
+    std::memset(&c, 0, sizeof(c));
+    c.qin = mode == 1 ? P0 : PO;
+    c.q0 = P0;
+    c.out = PO;
+    c.nfill = nph - (p->fuse_last ? 1 : 0);
+    c.zfuse = p->fuse_last ? 1 : 0;
+    c.ndt = mode == 1 && dt_out ? p->n_cases : 0;
+    c.head = ((c.ndt + wx * (int)p->wall.size()) + 7) / 8 * 8;
+    c.part = c.ndt ? p->d_part : nullptr;
+    c.dt_out = c.ndt ? dt_out : nullptr;
+    c.d_dt = d_dt;
+    c.dt_host = dt;
+    WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_stage(mode, sc, p->d_stage, c, &p->stage_grid, s); }));
+  }
+  WMX_CHECK(launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s));
+  return wmx_mass_flux(p, qout, stream);
+}
+
 int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
                  const double* d_dt, double* dt_out, double time, void* stream) {
   for (int b = 0; b < p->nb; ++b)
     if (q0[b] == qout[b]) return fail("qout must not alias q0");
+  if (p->persistent) return advance_persistent(p, q0, qout, dt, d_dt, dt_out, stream);
   hipStream_t s = as_stream(stream);
   Ptrs P0 = make_ptrs(p, q0), PO = make_ptrs(p, qout);
   const int ntiles = (int)p->tiles.size();
@@ -1056,7 +1220,7 @@ int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
 
 int wmx_advance(wmx_plan* p, double* const* q0, double* const* qout, double dt, const double* d_dt,
                 double time, void* stream) {
-  if (check_plan(p)) return 1;
+  if (check_plan(p) || refuse_external(p)) return 1;
   return advance_impl(p, q0, qout, dt, d_dt, nullptr, time, stream);
 }
 
@@ -1156,12 +1320,35 @@ int wmx_plan_set_tiling(wmx_plan* p, int32_t ci_len, int32_t order) {
   p->d_tiles = nullptr;
   if (upload(p->tiles, &p->d_tiles)) return 1;
   drop_graphs(p);
+  p->stage_dirty = true;
+  return 0;
+}
+
+int wmx_plan_set_persistent(wmx_plan* p, int32_t on) {
+  if (check_plan(p)) return 1;
+  if (!on) {
+    p->persistent = false;
+    return 0;
+  }
+  const std::string why = persistent_blocker(p);
+  if (!why.empty()) return fail(why);
+  if (!p->d_stage) {
+    std::vector<int> moff(p->phase_off.begin(), p->phase_off.end());
+    std::vector<int> mmax(p->phase_max.begin(), p->phase_max.end());
+    if (mmax.empty()) mmax.push_back(0);
+    WMX_CHECK(hipMalloc((void**)&p->d_stage, sizeof(StageStatic)));
+    WMX_CHECK(hipMalloc((void**)&p->d_pbuf, 3 * sizeof(Ptrs)));
+    if (upload(moff, &p->d_fill_off) || upload(mmax, &p->d_fill_max)) return 1;
+  }
+  drop_graphs(p);
+  p->stage_dirty = true;
+  p->persistent = true;
   return 0;
 }
 
 int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, double time,
              int32_t use_graph, void* stream) {
-  if (check_plan(p)) return 1;
+  if (check_plan(p) || refuse_external(p)) return 1;
   if (!d_dt) return fail("null d_dt");
   hipStream_t s = as_stream(stream);
   bool has_inflow = false;
@@ -1356,6 +1543,52 @@ int wmx_wall_net_eval(const wmx_numerics_desc* nm, const wmx_wall_net_desc* net,
   nd.net = make_netdev(net);
   WMX_CHECK(launch_wm_net_eval(nd, mode, n, h, hdel, U, T_h, p_h, T_wall, tau_w, q_w, uplus,
                                as_stream(stream)));
+  return 0;
+}
+
+int wmx_scheme_flux(const wmx_numerics_desc* nm, const int32_t n[3], int32_t d, const double* w,
+                    const double* S_d, const double* theta, double* F, void* stream) {
+  if (!nm || !n || !w || !S_d || !F) return fail("null argument");
+  if (d < 0 || d > 2) return fail("direction must be 0, 1 or 2");
+  if (n[0] < 3 || n[1] < 3 || n[2] < 3) return fail("each direction needs at least ng = 3 cells");
+  if (nm->scheme == WMX_SCHEME_HYBRID && !theta) return fail("the Hybrid scheme needs theta");
+  if (nm->scheme < WMX_SCHEME_KEP || nm->scheme > WMX_SCHEME_HYBRID) return fail("unknown scheme");
+  if ((nm->scheme != WMX_SCHEME_MUSCLHLLC) && nm->order != 2 && nm->order != 4) return fail("order must be 2 or 4");
+  NumDev nd = make_numdev(nm);
+  WMX_CHECK(launch_scheme_flux(nd, n, d, w, S_d, theta, F, as_stream(stream)));
+  return 0;
+}
+
+int wmx_sgs_eval(const wmx_numerics_desc* nm, int64_t n, const double* g, const double* delta,
+                 const double* rho, double* mu_t, void* stream) {
+  if (!nm || !g || !delta || !rho || !mu_t) return fail("null argument");
+  if (n < 0) return fail("n must be >= 0");
+  if (nm->sgs < WMX_SGS_NONE || nm->sgs > WMX_SGS_VREMAN) return fail("unknown SGS model");
+  NumDev nd = make_numdev(nm);
+  WMX_CHECK(launch_sgs_eval(nd, n, g, delta, rho, mu_t, as_stream(stream)));
+  return 0;
+}
+
+int wmx_wall_flux(const wmx_block_desc* blk, int32_t axis, int32_t side, const wmx_numerics_desc* nm,
+                  const double* w, const double* T, double* F, double* tau_w, double* q_w, void* stream) {
+  if (!blk || !nm || !w || !T || !F || !tau_w || !q_w) return fail("null argument");
+  if (axis < 0 || axis > 2 || side < 0 || side > 1) return fail("bad face");
+  const wmx_face_desc& f = blk->faces[2 * axis + side];
+  if (f.kind != WMX_BC_WALLMODEL) return fail("wmx_wall_flux needs a WallModel face");
+  if (f.n_match < 1 || f.n_match > blk->n[axis]) return fail("n_match out of range");
+  if (!blk->S[axis] || !blk->xf[axis] || !blk->xc) return fail("face vectors, face centres and centroids required");
+  const int ng = 3;
+  BlockDev B;
+  std::memset(&B, 0, sizeof(B));
+  for (int a = 0; a < 3; ++a) { B.n[a] = blk->n[a]; B.N[a] = blk->n[a] + 2 * ng; }
+  B.ng = ng;
+  B.cells = (long long)B.N[0] * B.N[1] * B.N[2];
+  B.vol = blk->vol;
+  B.xc = blk->xc;
+  for (int a = 0; a < 3; ++a) { B.S[a] = blk->S[a]; B.xf[a] = blk->xf[a]; }
+  for (int k = 0; k < 6; ++k) B.faces[k] = blk->faces[k];
+  NumDev nd = make_numdev(nm);
+  WMX_CHECK(launch_wall_flux(nd, B, axis, side, w, T, F, tau_w, q_w, as_stream(stream)));
   return 0;
 }
 

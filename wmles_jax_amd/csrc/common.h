@@ -178,6 +178,62 @@ hipError_t launch_wm_net_eval(const NumDev& nd, int mode, long long n, const dou
                               const double* ph, double T_wall, double* tau, double* qw,
                               double* uplus, hipStream_t s);
 hipError_t launch_mfma_probe(const float* A, const float* B, float* D, hipStream_t s);
+// Scheme configurations compiled as flux-kernel instances of their own (the face flux then
+// carries no runtime dispatch); SC_ANY reads everything from NumDev.
+enum { SC_ANY = 0, SC_HYB_KEP4_ALBADA = 1, SC_KEP2 = 2, SC_KEP4 = 3 };
+
+__host__ __device__ inline int scheme_config(const NumDev& nd) {
+  if (nd.scheme == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEP && nd.order != 2 && nd.limiter == 0)
+    return SC_HYB_KEP4_ALBADA;
+  if (nd.scheme == WMX_SCHEME_KEP) return nd.order == 2 ? SC_KEP2 : SC_KEP4;
+  return SC_ANY;
+}
+
+// One RK stage of an FP32-bulk plan as a single cooperative (persistent) launch
+// (stage.hip): the sweep's phases, the primitive records (+ CFL partials at stage 1), the
+// gradients with their head work (per-case dt min, Newton wall faces) and the flux tiles,
+// separated by grid-wide barriers.  What does not change between calls lives in device
+// memory (StageStatic, uploaded by the plan); the state pointers and the stage's own values
+// come as kernel arguments (StageCall), and the launch's workgroup 0 copies the pointer
+// tables to StageStatic::pbuf before the first barrier, so that the bodies index them in
+// global memory.
+struct StageStatic {
+  const BlockDev* blks;
+  NumDev nd;
+  const GhostOp* ops;          // sweep phases: ops[fill_off[ph] .. fill_off[ph + 1])
+  const int* fill_off;
+  const int* fill_max;         // cells of the phase's largest op
+  const int* pre_all;          // primitive workgroups per block (prefix)
+  int nblk, nprim;
+  const int* pre_box;          // gradient workgroups per block (prefix)
+  int ngrad;
+  const WallOp* wall_ops;      // Newton wall faces in the gradient head
+  int wall_wgx, nwall_ops;
+  const int* dt_list;          // per-case workgroup lists of the CFL partials
+  const int* dt_off;
+  const int4* tiles;
+  int ntiles, ci_len;
+  Ptrs* pbuf;                  // [3]: qin, q0, out of the running launch
+};
+struct StageCall {
+  Ptrs qin, q0, out;
+  int nfill, zfuse, head, ndt;
+  double* part;                // CFL partials (stage 1) or nullptr
+  double* dt_out;              // per-case dt (stage 1) or nullptr
+  const double* d_dt;
+  double dt_host;
+};
+// grid: workgroups of the persistent grid (0: the co-resident maximum, a multiple of 8,
+// returned in *grid_used)
+hipError_t launch_stage(int mode, int sc, const StageStatic* d_st, const StageCall& c, int* grid_used,
+                        hipStream_t s);
+
+hipError_t launch_scheme_flux(const NumDev& nd, const int* n, int d, const double* w, const double* S,
+                              const double* theta, double* F, hipStream_t s);
+hipError_t launch_sgs_eval(const NumDev& nd, long long n, const double* g, const double* delta,
+                           const double* rho, double* mut, hipStream_t s);
+hipError_t launch_wall_flux(const NumDev& nd, const BlockDev& B, int a, int side, const double* w,
+                            const double* T, double* F, double* tau, double* qw, hipStream_t s);
 int mlp_nparam();
 int mlp_nbf16();
 int mlp_nf32();

@@ -48,21 +48,7 @@ __global__ void k_fill_nodes(const BlockDev* __restrict__ blks, const GhostOp* _
 // ghost_value (ghostfill.h).
 __global__ void k_fill_cells(const BlockDev* __restrict__ blks, const GhostOp* __restrict__ ops,
                              Ptrs q, GasC gas) {
-  const GhostOp op = ops[blockIdx.y];
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= op.count) return;
-  const BlockDev& B = blks[op.blk];
-  const int ng = B.ng;
-  int m, c[3];
-  ghost_coords(idx, op.axis, ng, B.N, &m, c);
-  const int a = op.axis, s = op.side, n = B.n[a];
-  c[a] = s == 1 ? ng + n + m : ng - 1 - m;
-  double out[5];
-  if (!ghost_value(blks, op.blk, a, s, m, c, q, gas, out)) return;   // inflow: k_inflow
-  double* qd = q.p[op.blk];
-  const long long cells = B.cells;
-  const long long di = cidx(B.N, c[0], c[1], c[2]);
-  for (int v = 0; v < 5; ++v) qd[v * cells + di] = out[v];
+  fill_cell(blks, ops[blockIdx.y], q, gas, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // Face vectors and face centres of direction d, reference mesh.py:58-65.

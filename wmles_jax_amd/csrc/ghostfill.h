@@ -45,7 +45,7 @@ __device__ __forceinline__ void interface_src(const wmx_face_desc& f, int axis, 
 // Value of ghost cell c (c[a] already the ghost coordinate, m its layer off the face) of
 // block blk for face (a, s), into out[5].  false for inflow faces (k_inflow fills them).
 __device__ __forceinline__ bool ghost_value(const BlockDev* __restrict__ blks, int blk, int a,
-                                            int s, int m, const int* c, Ptrs q, const GasC& gas,
+                                            int s, int m, const int* c, const Ptrs& q, const GasC& gas,
                                             double* out) {
   const BlockDev& B = blks[blk];
   const int ng = B.ng, n = B.n[a];
@@ -90,6 +90,24 @@ __device__ __forceinline__ bool ghost_value(const BlockDev* __restrict__ blks, i
       bc_slip(gas, src, nx, ny, nz, out);
   }
   return true;
+}
+
+// Ghost cell idx of fill op `op` (one thread per ghost cell of the op's slab).
+__device__ __forceinline__ void fill_cell(const BlockDev* __restrict__ blks, const GhostOp& op, const Ptrs& q,
+                                          const GasC& gas, int idx) {
+  if (idx >= op.count) return;
+  const BlockDev& B = blks[op.blk];
+  const int ng = B.ng;
+  int m, c[3];
+  ghost_coords(idx, op.axis, ng, B.N, &m, c);
+  const int a = op.axis, s = op.side, n = B.n[a];
+  c[a] = s == 1 ? ng + n + m : ng - 1 - m;
+  double out[5];
+  if (!ghost_value(blks, op.blk, a, s, m, c, q, gas, out)) return;   // inflow: k_inflow
+  double* qd = q.p[op.blk];
+  const long long cells = B.cells;
+  const long long di = cidx(B.N, c[0], c[1], c[2]);
+  for (int v = 0; v < 5; ++v) qd[v * cells + di] = out[v];
 }
 
 }  // namespace wmx

@@ -5,91 +5,9 @@
 // (gas.py:24-33), whose stage-1 launch of a step can take the CFL partials, so the
 // separate dt kernel and the fused one evaluate the same code.  Built with
 // -ffp-contract=off.
-#include "ghostfill.h"
+#include "prim32.h"
 
 namespace wmx {
-
-__device__ __forceinline__ int find_block_r(const int* __restrict__ prefix, int nblk, int wg) {
-  int b = 0;
-  while (b + 1 < nblk && prefix[b + 1] <= wg) ++b;
-  return b;
-}
-
-// NaN-propagating min (numpy.min semantics)
-__device__ __forceinline__ double nmin(double a, double b) {
-  if (a != a) return a;
-  if (b != b) return b;
-  return a < b ? a : b;
-}
-
-template <bool MIN>
-__device__ double block_reduce(double v) {
-  __shared__ double red[16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int o = 32; o > 0; o >>= 1) {
-    double u = __shfl_xor(v, o, 64);
-    v = MIN ? nmin(v, u) : v + u;
-  }
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  const int nw = (blockDim.x + 63) >> 6;
-  double r = red[0];
-  for (int w = 1; w < nw; ++w) r = MIN ? nmin(r, red[w]) : r + red[w];
-  __syncthreads();
-  return r;
-}
-
-// interior cell l of block B -> ghosted linear index (32-bit: plans keep every plane
-// below 2^29 cells)
-__device__ __forceinline__ unsigned interior_cell(const BlockDev& B, unsigned l, int* I, int* J,
-                                                  int* K) {
-  const unsigned n1 = (unsigned)B.n[1], n2 = (unsigned)B.n[2], ng = (unsigned)B.ng;
-  const unsigned r = l / n2;
-  *K = (int)(ng + (l - r * n2));
-  *J = (int)(ng + r % n1);
-  *I = (int)(ng + r / n1);
-  return ((unsigned)*I * (unsigned)B.N[1] + (unsigned)*J) * (unsigned)B.N[2] + (unsigned)*K;
-}
-
-// CFL time step of interior cell (I, J, K) (ghosted index c) from its primitives w;
-// solver.py:139-150.  Every load first (one memory round trip), then the arithmetic.  The
-// dt kernel and the fused stage-1 primitive kernels below share it in this unit, so the
-// two give the same bits.
-__device__ __forceinline__ double cell_dt(const BlockDev& B, const NumDev& nd, unsigned c, int I,
-                                          int J, int K, const Prim& w) {
-  double Sv[3][3][2];
-  const double vol = ldu(B.vol, c);
-  const int cc[3] = {I, J, K};
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double* S = B.S[d];
-    const unsigned fc = (unsigned)fcount(B.N, d);
-    int hi[3] = {cc[0], cc[1], cc[2]};
-    hi[d] += 1;
-    const unsigned f0 = (unsigned)fidx(B.N, d, cc[0], cc[1], cc[2]);
-    const unsigned f1 = (unsigned)fidx(B.N, d, hi[0], hi[1], hi[2]);
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      Sv[d][x][0] = ldu(S + x * fc, f0);
-      Sv[d][x][1] = ldu(S + x * fc, f1);
-    }
-  }
-  const GasC& gas = nd.gas;
-  const double cs = sqrt(gas.gamma * gas.R * w.T);
-  const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
-  const double nu = (cnu * mu_suth(gas, w.T)) / w.r;
-  double lam = 0.0, lamv = 0.0;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double s0 = 0.5 * (Sv[d][0][0] + Sv[d][0][1]);
-    const double s1 = 0.5 * (Sv[d][1][0] + Sv[d][1][1]);
-    const double s2 = 0.5 * (Sv[d][2][0] + Sv[d][2][1]);
-    const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
-    lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
-    lamv = lamv + (nu * (Sm * Sm)) / vol;
-  }
-  return nd.cfl / ((lam + 4.0 * lamv) / vol);
-}
 
 // Per-cell CFL time step, min over interiors; solver.py:139-150.
 __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__ blks,
@@ -120,104 +38,6 @@ __global__ __launch_bounds__(256) void k_dt_partial(const BlockDev* __restrict__
   if (threadIdx.x == 0) part[wg] = r;
 }
 
-// Spanwise mean of sponge row `row` for rows of 9..129 cells (numpy's pairwise order for
-// the elements 1..nk-1 by eight lanes u of the row, combined by shuffles in that order);
-// every lane of the 8-lane group calls it.  smooth_ramp.py:214-216.
-__device__ __forceinline__ void sponge_row_fast(const BlockDev& B, long long row, int u, bool live,
-                                                long long c0, const double* __restrict__ qb) {
-  const long long rows = (long long)B.n[0] * B.n[1];
-  const int n = B.n[2] - 1;
-  const double* vol = B.vol + c0;
-  const long long cells = B.cells;
-  double a[6];
-  const int nfull = n - n % 8;
-  if (live) {
-    const double vv = vol[1 + u];
-    a[0] = vv;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) a[1 + v] = qb[v * cells + c0 + 1 + u] * vv;
-    for (int i = 8; i < nfull; i += 8) {
-      const double w = vol[1 + i + u];
-      a[0] = a[0] + w;
-#pragma unroll
-      for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i + u] * w;
-    }
-  } else {
-#pragma unroll
-    for (int v = 0; v < 6; ++v) a[v] = 0.0;
-  }
-#pragma unroll
-  for (int m = 1; m <= 4; m <<= 1)
-#pragma unroll
-    for (int v = 0; v < 6; ++v) a[v] = a[v] + __shfl_xor(a[v], m, 8);
-  if (!live || u != 0) return;
-  for (int i = nfull; i < n; ++i) {
-    const double w = vol[1 + i];
-    a[0] = a[0] + w;
-    for (int v = 0; v < 5; ++v) a[1 + v] = a[1 + v] + qb[v * cells + c0 + 1 + i] * w;
-  }
-  const double den = vol[0] + a[0];
-  for (int v = 0; v < 5; ++v) B.sponge_mean[v * rows + row] = (qb[v * cells + c0] * vol[0] + a[1 + v]) / den;
-}
-
-// Head workgroups of a primitive launch (a multiple of 8, so the primitive workgroups keep
-// their XCD spans): workgroup h < sh.nwg takes 32 sponge rows of the block whose range
-// sh.pre[b] <= h < sh.pre[b + 1] holds it (rows of 9..129 cells only).
-__device__ __forceinline__ void sponge_head(const BlockDev* __restrict__ blks, int nblk, Ptrs q,
-                                            const SpongeHead& sh) {
-  const int h = (int)blockIdx.x;
-  if (h >= sh.nwg) return;
-  int b = 0;
-  while (b + 1 < nblk && sh.pre[b + 1] <= h) ++b;
-  const BlockDev& B = blks[b];
-  const long long rows = (long long)B.n[0] * B.n[1];
-  const long long row = (long long)(h - sh.pre[b]) * 32 + (threadIdx.x >> 3);
-  const int u = threadIdx.x & 7;
-  const bool live = row < rows && B.sponge_row[row];
-  const int ng = B.ng;
-  const long long c0 = live ? cidx(B.N, ng + (int)(row / B.n[1]), ng + (int)(row % B.n[1]), ng) : 0;
-  sponge_row_fast(B, row, u, live, c0, q.p[b]);
-}
-
-// ------------------------------------------------------------------------------ prim
-// Primitive records of every cell (ghosts included).  DT: the stage-1 launch of a step
-// also takes the CFL partial of its workgroup's interior cells into part[workgroup],
-// reusing the state it loads (the per-case min then follows as for k_dt_partial).
-__device__ __forceinline__ bool interior_of(const BlockDev& B, unsigned c, int* I, int* J, int* K) {
-  const unsigned N1 = (unsigned)B.N[1], N2 = (unsigned)B.N[2];
-  const unsigned r = c / N2;
-  *K = (int)(c - r * N2);
-  *J = (int)(r % N1);
-  *I = (int)(r / N1);
-  const int ng = B.ng;
-  return *I >= ng && *I < ng + B.n[0] && *J >= ng && *J < ng + B.n[1] && *K >= ng &&
-         *K < ng + B.n[2];
-}
-
-// State of cell c: from q, or (zfuse) for a k ghost of a flagged side the value the sweep's
-// last phase would store there (ghost_value, the fill kernel's own code)
-__device__ __forceinline__ void prim_state(const BlockDev* __restrict__ blks, int b, long long c,
-                                           Ptrs q, const GasC& gas, int zfuse, double* qv) {
-  const BlockDev& B = blks[b];
-  if (zfuse && B.zfuse) {
-    // 32-bit decode (every block holds fewer than 2^32 cells); I, J only for the ghosts
-    const unsigned N2 = (unsigned)B.N[2];
-    const unsigned r = (unsigned)c / N2;
-    const int K = (int)((unsigned)c - r * N2), ng = B.ng, n2 = B.n[2];
-    const int side = K < ng ? 0 : (K >= ng + n2 ? 1 : -1);
-    if (side >= 0 && ((B.zfuse >> side) & 1)) {
-      const unsigned N1 = (unsigned)B.N[1];
-      const int cc[3] = {(int)(r / N1), (int)(r % N1), K};
-      const int m = side == 0 ? ng - 1 - K : K - ng - n2;
-      if (ghost_value(blks, b, 2, side, m, cc, q, gas, qv)) return;
-    }
-  }
-  const double* qb = q.p[b];
-  const long long cells = B.cells;
-#pragma unroll
-  for (int v = 0; v < 5; ++v) qv[v] = qb[v * cells + c];
-}
-
 // FP64 record (r, u, v, w, p, T, mu, e_int)
 template <bool DT>
 __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
@@ -225,7 +45,7 @@ __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
                                               Ptrs q, NumDev nd, double* __restrict__ part,
                                               int zfuse, SpongeHead sh) {
   if ((int)blockIdx.x < sh.head) {
-    sponge_head(blks, nblk, q, sh);
+    sponge_head(blks, nblk, q, sh, (int)blockIdx.x);
     return;
   }
   // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
@@ -259,41 +79,12 @@ __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
   }
 }
 
-// FP32-bulk record: prims evaluated in FP64, stored in FP32 with p, T and e_int
-// shifted by the block gauge so their small variations keep FP32 precision.
 template <bool DT>
 __global__ __launch_bounds__(256) void k_prim32(const BlockDev* __restrict__ blks,
                                                 const int* __restrict__ wg_prefix, int nblk,
                                                 Ptrs q, NumDev nd, double* __restrict__ part,
                                                 int zfuse, SpongeHead sh) {
-  if ((int)blockIdx.x < sh.head) {
-    sponge_head(blks, nblk, q, sh);
-    return;
-  }
-  // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice
-  // stay in one L2
-  const int wg = xcd_span((int)blockIdx.x - sh.head, (int)gridDim.x - sh.head);
-  const int b = find_block_r(wg_prefix, nblk, wg);
-  const BlockDev& B = blks[b];
-  const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  const long long cells = B.cells;
-  double dtc = __builtin_huge_val();
-  if (c < cells) {
-    const GasC& gas = nd.gas;
-    double qv[5];
-    prim_state(blks, b, c, q, gas, zfuse, qv);
-    const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
-    st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - B.T_gauge)));
-    st4(B.Pf + 4 * ((size_t)cells + c),
-        make_float4((float)w.r, (float)(w.p - B.p_ref), (float)mu_suth(gas, w.T),
-                    (float)(w.p / ((gas.gamma - 1.0) * w.r) - B.e_ref)));
-    int I, J, K;
-    if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
-  }
-  if (DT) {
-    const double r = block_reduce<true>(dtc);
-    if (threadIdx.x == 0) part[wg] = r;
-  }
+  prim32_block<DT>(blks, wg_prefix, nblk, q, nd, part, zfuse, sh, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // sum over interiors of rho u_dir vol (and vol); solver.py:23-27

@@ -8,17 +8,10 @@
 //   (the primitive kernels, q -> rho u v w p T mu e_int for all ghosted cells (gas.py:24-33),
 //   are in reduce.hip with the fused CFL partial; the fused face-flux / divergence / stage
 //   kernel is in flux.hip)
-#include "common.h"
-#include "wallnet.h"
+#include "grad32.h"
 
 namespace wmx {
 
-
-__device__ __forceinline__ int find_block(const int* __restrict__ prefix, int nblk, int wg) {
-  int b = 0;
-  while (b + 1 < nblk && prefix[b + 1] <= wg) ++b;
-  return b;
-}
 
 // ------------------------------------------------------------------------------ grad
 // FP64: one thread per cell.
@@ -101,123 +94,6 @@ __global__ __launch_bounds__(256) void k_grad_cell(const BlockDev* __restrict__ 
   G[G_TH * cells + c] = ducros<R>(g);
 }
 
-// Wall flux, tau_w and q_w of face idx0 of wall op `op` (bc.py:145-166); NET: the learned
-// model with its weights staged in sW / sF, idx0 past the count clamped for the network.
-template <bool NET>
-__device__ __forceinline__ void wall_face(const BlockDev* __restrict__ blks, const WallOp& op,
-                                          int idx0, Ptrs q, const NumDev& nd,
-                                          const unsigned short* sW, const float* sF) {
-  const int idx = NET ? min(idx0, op.count - 1) : idx0;
-  const BlockDev& B = blks[op.blk];
-  const int a = op.axis, s = op.side, ng = B.ng, n = B.n[a];
-  const wmx_face_desc& f = B.faces[2 * a + s];
-  int t1, t2;
-  tangents(a, &t1, &t2);
-  int c[3];
-  c[t2] = ng + idx % B.n[t2];
-  c[t1] = ng + idx / B.n[t2];
-  const int fb = s == 0 ? ng : ng + n;
-  const int mi = s == 0 ? ng + f.n_match - 1 : ng + n - f.n_match;
-  const int ci = s == 0 ? ng : ng + n - 1;
-  const int* N = B.N;
-  const long long cells = B.cells;
-  int cf[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]}, cb[3] = {c[0], c[1], c[2]};
-  cf[a] = fb; cm[a] = mi; cb[a] = ci;
-  const long long fi = fidx(N, a, cf[0], cf[1], cf[2]), fc = fcount(N, a);
-  const double* S = B.S[a];
-  const double* xf = B.xf[a];
-  double Sv[3] = {S[fi], S[fc + fi], S[2 * fc + fi]};
-  double Sm = sqrt((Sv[0] * Sv[0] + Sv[1] * Sv[1]) + Sv[2] * Sv[2]);
-  double no[3];
-  for (int x = 0; x < 3; ++x) no[x] = s == 1 ? Sv[x] / Sm : -Sv[x] / Sm;
-  const long long m = cidx(N, cm[0], cm[1], cm[2]);
-  const long long cbi = cidx(N, cb[0], cb[1], cb[2]);
-  double h = 0.0;
-  {
-    double d0 = (B.xc[m] - xf[fi]) * (-no[0]);
-    double d1 = (B.xc[cells + m] - xf[fc + fi]) * (-no[1]);
-    double d2 = (B.xc[2 * cells + m] - xf[2 * fc + fi]) * (-no[2]);
-    h = (d0 + d1) + d2;
-  }
-  const double* qb = q.p[op.blk];
-  const GasC& gas = nd.gas;
-  Prim wm = cons_to_prim(gas, qb[m], qb[cells + m], qb[2 * cells + m], qb[3 * cells + m],
-                         qb[4 * cells + m]);
-  Prim wb = cons_to_prim(gas, qb[cbi], qb[cells + cbi], qb[2 * cells + cbi], qb[3 * cells + cbi],
-                         qb[4 * cells + cbi]);
-  double un = dot3(wm.u, wm.v, wm.w, no[0], no[1], no[2]);
-  double up[3] = {wm.u - un * no[0], wm.v - un * no[1], wm.w - un * no[2]};
-  double U = sqrt(((up[0] * up[0] + up[1] * up[1]) + up[2] * up[2]) + 1e-20);
-  double tau, qw;
-  if (NET) {
-    double up;
-    const bool fb = learned_wall(nd, sW, sF, 0, h, h / cbrt(B.vol[m]), U, wm.T, wm.p, f.T_wall,
-                                 &tau, &qw, &up);
-    const unsigned long long bal = __ballot(fb && idx0 < op.count);
-    if (nd.net.nfb && (threadIdx.x & 63) == 0 && bal) atomicAdd(nd.net.nfb, (unsigned int)__popcll(bal));
-    if (idx0 >= op.count) return;
-  } else {
-    equilibrium_wall(gas, nd.wm, h, U, wm.T, wm.p, f.T_wall, &tau, &qw);
-  }
-  double sg = s == 1 ? 1.0 : -1.0;
-  double* W = B.wall[2 * a + s];
-  const long long nw = (long long)B.n[t1] * B.n[t2];
-  // FP32-bulk plans take the wall flux in gauge form: (p - p_ref) instead of p (the
-  // p_ref S terms of a cell's faces telescope; no mass flux, so no energy gauge term)
-  const double pw = nd.prec == WMX_PREC_FP32_BULK ? wb.p - B.p_ref : wb.p;
-  W[idx] = sg * 0.0;
-  for (int x = 0; x < 3; ++x) W[(1 + x) * nw + idx] = sg * ((pw * no[x] + tau * (up[x] / U)) * Sm);
-  W[4 * nw + idx] = sg * ((-qw) * Sm);
-  W[5 * nw + idx] = tau;
-  W[6 * nw + idx] = qw;
-}
-// FP32-bulk gradient / SGS / Ducros.  One thread per run of grad_run(4) consecutive i
-// cells of one (j, k) column: the i neighbours' phi and the shared i-face vectors stay in
-// registers from one cell of the run to the next, so the planes i-1 / i+1 (far apart in
-// memory) are read about once instead of three times.  Per cell the arithmetic is that
-// of a single-cell evaluation.  Addressing: every plane / face-component array gets a
-// wave-uniform base pointer and each load a 32-bit byte offset from it (the plan checks
-// that every plane stays below 4 GiB), so loads take the scalar-base form and need one
-// 32-bit offset register instead of a 64-bit address pair each.  The SGS model is a
-// template parameter so that only one model is compiled into each instance.
-// NaN-propagating min (numpy.min semantics)
-__device__ __forceinline__ double nan_min(double a, double b) {
-  if (a != a) return a;
-  if (b != b) return b;
-  return a < b ? a : b;
-}
-
-// dt of case cs: the min of the case's CFL partials (list == nullptr: the identity list),
-// one 256-thread workgroup, eight independent loads in flight per thread
-__device__ void case_dt_min(const int* __restrict__ list, const int* __restrict__ off,
-                            const double* __restrict__ part, int cs, double* __restrict__ out) {
-  __shared__ double red[4];
-  const int lo = off[cs], hi = off[cs + 1];
-  double a = __builtin_huge_val();
-  for (int t0 = lo + (int)threadIdx.x; t0 < hi; t0 += 8 * (int)blockDim.x) {
-    double va[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int t = t0 + u * (int)blockDim.x;
-      va[u] = t < hi ? part[list ? list[t] : t] : __builtin_huge_val();
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a = nan_min(a, va[u]);
-  }
-  for (int o = 32; o > 0; o >>= 1) a = nan_min(a, __shfl_xor(a, o, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double r = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = nan_min(r, red[w]);
-    out[cs] = r;
-  }
-}
-
-// The launch's first `head` workgroups (a multiple of 8, so the gradient workgroups keep
-// their XCD spans) do latency-bound side work beside the memory-bound gradients:
-//   [0, ndt): the per-case dt min of the CFL partials (the step's first stage);
-//   [ndt, ndt + wall_wgx * nwall_ops): the Newton wall faces, wall_wgx per wall op.
 template <int SGS>
 __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blks,
                                                 const int* __restrict__ wg_prefix, int nblk,
@@ -227,148 +103,8 @@ __global__ __launch_bounds__(256) void k_grad32(const BlockDev* __restrict__ blk
                                                 const int* __restrict__ dt_off,
                                                 const double* __restrict__ dt_part,
                                                 double* __restrict__ dt_out) {
-  if ((int)blockIdx.x < head) {
-    const int h = (int)blockIdx.x;
-    if (h < ndt) {
-      case_dt_min(dt_list, dt_off, dt_part, h, dt_out);
-      return;
-    }
-    const int op_i = (h - ndt) / max(wall_wgx, 1);
-    if (wall_wgx <= 0 || op_i >= nwall_ops) return;      // padding workgroups
-    const WallOp op = wall_ops[op_i];
-    const int idx0 = (h - ndt - op_i * wall_wgx) * (int)blockDim.x + (int)threadIdx.x;
-    if (idx0 < op.count) wall_face<false>(blks, op, idx0, q, nd, nullptr, nullptr);
-    return;
-  }
-  const int wg = xcd_span((int)blockIdx.x - head, (int)gridDim.x - head);
-  const int b = find_block(wg_prefix, nblk, wg);
-  const BlockDev& B = blks[b];
-  const int ng = B.ng;
-  const int b0 = B.n[0] + 2, b1 = B.n[1] + 2, b2 = B.n[2] + 2;
-  constexpr int RUN = grad_run(sizeof(float));
-  const int nrun = (b0 + RUN - 1) / RUN;
-  const int l = (wg - wg_prefix[b]) * (int)blockDim.x + (int)threadIdx.x;
-  if (l >= nrun * b1 * b2) return;
-  const int K = ng - 1 + l % b2;
-  const int J = ng - 1 + (l / b2) % b1;
-  const int Ia = ng - 1 + (l / (b1 * b2)) * RUN;
-  const int Ib = min(Ia + RUN, ng + B.n[0] + 1);
-  const int N1 = B.N[1], N2 = B.N[2];
-  const unsigned cells = (unsigned)B.cells;
-  const unsigned st0 = (unsigned)N1 * N2, st1 = (unsigned)N2;
-  // face-array strides: S[d] has N_d + 1 entries along d
-  const unsigned f0s0 = st0;                              // i faces: (I, J, K) -> I N1 N2 + J N2 + K
-  const unsigned f1s0 = (unsigned)(N1 + 1) * N2;          // j faces: M1 = N1 + 1
-  const unsigned f2s0 = (unsigned)N1 * (N2 + 1), f2s1 = (unsigned)(N2 + 1);
-  const unsigned fc0 = (unsigned)fcount(B.N, 0), fc1 = (unsigned)fcount(B.N, 1),
-                 fc2 = (unsigned)fcount(B.N, 2);
-  const float* Pf = B.Pf;
-  // (u, v, w, T') of a cell: the first 16-byte group of its primitive record
-  auto phi = [&](unsigned cc, float* o) {
-    const float4 a = ld4(Pf + 4 * (size_t)cc);
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-  };
-  // float copies of the face vectors and volumes (the arithmetic is FP32; same values)
-  const float* S0[3] = {B.Sf[0], B.Sf[0] + fc0, B.Sf[0] + 2 * (size_t)fc0};
-  const float* S1[3] = {B.Sf[1], B.Sf[1] + fc1, B.Sf[1] + 2 * (size_t)fc1};
-  const float* S2[3] = {B.Sf[2], B.Sf[2] + fc2, B.Sf[2] + 2 * (size_t)fc2};
-  float* Gf = B.Gf;
-  const float h = 0.5f;
-  const GasC& gs = nd.gas;
-  unsigned c = (unsigned)Ia * st0 + (unsigned)J * st1 + (unsigned)K;
-  unsigned f0 = c;                                        // i-face index of face (I, J, K)
-  unsigned f1 = (unsigned)Ia * f1s0 + (unsigned)J * st1 + (unsigned)K;
-  unsigned f2 = (unsigned)Ia * f2s0 + (unsigned)J * f2s1 + (unsigned)K;
-  float phm[4], ph0[4], S0lo[3];
-  phi(c - st0, phm);
-  phi(c, ph0);
-#pragma unroll
-  for (int x = 0; x < 3; ++x) S0lo[x] = ldu(S0[x], f0);
-#pragma unroll 1
-  for (int I = Ia; I < Ib; ++I) {
-    float php[4], S0hi[3];
-    phi(c + st0, php);
-#pragma unroll
-    for (int x = 0; x < 3; ++x) S0hi[x] = ldu(S0[x], f0 + f0s0);
-    float acc[12];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float flo_v = h * (phm[v] + ph0[v]);
-      const float fhi_v = h * (ph0[v] + php[v]);
-#pragma unroll
-      for (int x = 0; x < 3; ++x) acc[3 * v + x] = fhi_v * S0hi[x] - flo_v * S0lo[x];
-    }
-    {
-      float Slo[3], Shi[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        Slo[x] = ldu(S1[x], f1);
-        Shi[x] = ldu(S1[x], f1 + st1);
-      }
-      float pm[4], pp[4];
-      phi(c - st1, pm);
-      phi(c + st1, pp);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float flo_v = h * (pm[v] + ph0[v]);
-        const float fhi_v = h * (ph0[v] + pp[v]);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
-      }
-    }
-    {
-      float Slo[3], Shi[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        Slo[x] = ldu(S2[x], f2);
-        Shi[x] = ldu(S2[x], f2 + 1);
-      }
-      float pm[4], pp[4];
-      phi(c - 1, pm);
-      phi(c + 1, pp);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float flo_v = h * (pm[v] + ph0[v]);
-        const float fhi_v = h * (ph0[v] + pp[v]);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) acc[3 * v + x] = acc[3 * v + x] + (fhi_v * Shi[x] - flo_v * Slo[x]);
-      }
-    }
-    const float volr = ldu((const float*)B.volf, c);
-    float g[12];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) g[i] = fdiv<float>(acc[i], volr);
-    const float4 pb = ld4(Pf + 4 * ((size_t)cells + c));     // (rho, p', mu, e')
-    const float rho = pb.x, mu = pb.z;
-    float mut;
-    if constexpr (SGS != WMX_SGS_NONE) {
-      // filter width vol^(1/3) (solver.py:82), single-precision cube root
-      const float delta = cbrtf(volr);
-      const float cs = (float)nd.sgs_c;
-      float nut;
-      if constexpr (SGS == WMX_SGS_SMAGORINSKY) nut = nut_smagorinsky<float>(g, delta, cs);
-      else if constexpr (SGS == WMX_SGS_WALE) nut = nut_wale<float>(g, delta, cs);
-      else nut = nut_vreman<float>(g, delta, cs);
-      mut = rho * nut;
-    } else {
-      mut = 0.0f * rho;
-    }
-    const size_t gq = 4 * (size_t)cells;                  // one float4 plane
-    float* grec = Gf + 4 * (size_t)c;
-    st4(grec, make_float4(g[0], g[1], g[2], g[3]));
-    st4(grec + gq, make_float4(g[4], g[5], g[6], g[7]));
-    st4(grec + 2 * gq, make_float4(g[8], g[9], g[10], g[11]));
-    st4(grec + 3 * gq, make_float4(mu + mut, (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t)),
-                                   ducros<float>(g), 0.0f));
-#pragma unroll
-    for (int v = 0; v < 4; ++v) { phm[v] = ph0[v]; ph0[v] = php[v]; }
-#pragma unroll
-    for (int x = 0; x < 3; ++x) S0lo[x] = S0hi[x];
-    c += st0;
-    f0 += f0s0;
-    f1 += f1s0;
-    f2 += f2s0;
-  }
+  grad32_block<SGS>(blks, wg_prefix, nblk, nd, head, wall_ops, wall_wgx, nwall_ops, q, ndt, dt_list, dt_off,
+                    dt_part, dt_out, (int)blockIdx.x, (int)gridDim.x);
 }
 template __global__ void k_grad32<WMX_SGS_NONE>(const BlockDev*, const int*, int, NumDev, int,
                                                   const WallOp*, int, int, Ptrs, int, const int*,

@@ -51,6 +51,8 @@ class EnsembleSolver:
         if not 1 <= len(solvers) <= 16:
             raise ValueError("an ensemble holds 1..16 cases")
         s0 = solvers[0]
+        if any(s._external for s in solvers):
+            raise ValueError("ensemble cases must not have user (Python) boundary conditions")
         for s in solvers[1:]:
             same = (s.gas == s0.gas and _numerics_key(s._nd) == _numerics_key(s0._nd)
                     and s.precision == s0.precision and not s.sources and not s0.sources)

@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from .backend import asarray, device
-from .bc import WallModel
+from .bc import WallModel, is_external
 from .mesh import block_desc
 from .schemes import KEP, KEEP, MUSCLHLLC, Hybrid
 from .sgs import SGSModel, make_sgs
@@ -105,7 +105,7 @@ def numerics_desc(gas, scheme, sgs, cfl, bc_passes, constraints, sources, n_case
     for src in sources:
         from .smooth_ramp import SmoothRampSponge
         if not isinstance(src, SmoothRampSponge):
-            raise NotImplementedError(f"source {type(src).__name__} has no HIP implementation")
+            continue                     # a user source: added by the Solver (torch)
         if nd.n_sponge >= 4:
             raise NotImplementedError("at most 4 sponge sources")
         vals = (src.x_start_l, src.x_end_l, src.strength, src.props["U_inf"], src.props["L"])
@@ -176,6 +176,19 @@ class Solver:
         _lib.check(_lib.lib.wmx_plan_wall_faces(plan, C.byref(cnt), rec))
         self._wall_faces = [tuple(rec[5 * i:5 * i + 5]) for i in range(cnt.value)]
         self._dt_buf = torch.empty(16, dtype=torch.float64, device=self._dev)
+        # user BCs (no HIP descriptor) and user sources run on the torch side: the sweep
+        # is then driven phase by phase from here, and the RK3 stages combine in torch
+        from .smooth_ramp import SmoothRampSponge
+        self._py_sources = [src for src in self.sources if not isinstance(src, SmoothRampSponge)]
+        self._external = any(is_external(bc) for b in self.blocks for bc in b.faces.values())
+        self._fallback = self._external or bool(self._py_sources)
+        self._phases = []
+        if self._external:
+            npd = C.c_int32()
+            _lib.check(_lib.lib.wmx_plan_sweep_phases(plan, C.byref(npd), None))
+            rec = (C.c_int32 * (4 * max(1, npd.value)))()
+            _lib.check(_lib.lib.wmx_plan_sweep_phases(plan, C.byref(npd), rec))
+            self._phases = [tuple(rec[4 * i:4 * i + 4]) for i in range(npd.value)]
         if gauge is not None:
             self.set_gauge(*gauge)
         from .learned_wm import LearnedWallModel
@@ -266,17 +279,80 @@ class Solver:
     def apply_bcs(self, Q, geoms=None, time=0.0):
         """Sequential ghost sweep (solver.py:59-73); returns new tensors."""
         out = tuple(q.clone() for q in self._as_state(Q))
+        if self._external:
+            return self._sweep(list(out), float(time))
         _lib.check(_lib.lib.wmx_apply_bcs(self._plan, _lib.ptr_array(out), float(time), _lib.stream_ptr()))
         return out
 
+    def _sweep(self, Q, time):
+        """The sweep with user BCs: runs of HIP phases between the user fills, each user
+        fill at its place in the reference's order (solver.py:59-73)."""
+        first = 0
+        for i, (ext, b, a, side) in enumerate(self._phases + [(1, -1, -1, -1)]):
+            if not ext:
+                continue
+            if i > first:
+                _lib.check(_lib.lib.wmx_apply_bcs_phases(self._plan, _lib.ptr_array(Q), first, i, time,
+                                                         _lib.stream_ptr()))
+            first = i + 1
+            if b < 0:
+                break
+            blk = self.blocks[b]
+            q = blk.faces[(a, side)].fill_at_time(Q[b], blk, self.geoms[b], self.gas, a, side, time)
+            q = asarray(q)
+            if tuple(q.shape) != tuple(Q[b].shape):
+                raise ValueError(f"{type(blk.faces[(a, side)]).__name__}.fill returned shape {tuple(q.shape)}")
+            Q[b] = q if q.data_ptr() != Q[b].data_ptr() else q.clone()
+        return tuple(Q)
+
     def rhs(self, Q, time=0.0):
         """RHS with zero ghosts (solver.py:97-106,159-160)."""
+        if self._fallback:
+            return self._rhs_torch(self._as_state(Q), float(time))
         work = tuple(q.clone() for q in self._as_state(Q))
         self._ensure_gauge(work)
         R = tuple(torch.empty_like(q) for q in work)
         _lib.check(_lib.lib.wmx_rhs(self._plan, _lib.ptr_array(work), _lib.ptr_array(R), float(time),
                                     _lib.stream_ptr()))
         return R
+
+    def _rhs_torch(self, Q, time):
+        """RHS for plans with user BCs or sources: the sweep (HIP phases and user fills),
+        the HIP RHS of the filled state, plus each user source on the interior
+        (solver.py:97-106)."""
+        Qf = self.apply_bcs(Q, None, time)
+        self._ensure_gauge(Qf)
+        R = tuple(torch.empty_like(q) for q in Qf)
+        _lib.check(_lib.lib.wmx_rhs_filled(self._plan, _lib.ptr_array(Qf), _lib.ptr_array(R), _lib.stream_ptr()))
+        for blk, q, r, g in zip(self.blocks, Qf, R, self.geoms):
+            for src in self._py_sources:
+                add = asarray(src(blk, q, g, self.gas))
+                interior = blk.interior(r)
+                if tuple(add.shape) != tuple(interior.shape):
+                    raise ValueError(f"source {type(src).__name__} returned {tuple(add.shape)}, "
+                                     f"expected {tuple(interior.shape)}")
+                interior += add
+        return R
+
+    def _advance_torch(self, Q, dt, time):
+        """SSP-RK3 of solver.py:109-133 with the stage combinations in torch (plans with
+        user BCs or sources), then the constraints."""
+        if not isinstance(dt, torch.Tensor):
+            dt = torch.tensor(float(dt), dtype=torch.float64, device=self._dev)
+        dt = dt.reshape(-1)[0] if dt.numel() > 1 else dt.reshape(())
+        tdt = float(dt)
+        R0 = self._rhs_torch(Q, time)
+        Q1 = tuple(q + dt * r for q, r in zip(Q, R0))
+        R1 = self._rhs_torch(Q1, time + tdt)
+        Q2 = tuple(0.75 * q0 + 0.25 * (q1 + dt * r1) for q0, q1, r1 in zip(Q, Q1, R1))
+        R2 = self._rhs_torch(Q2, time + 0.5 * tdt)
+        out = tuple((1.0 / 3.0) * q0 + (2.0 / 3.0) * (q2 + dt * r2) for q0, q2, r2 in zip(Q, Q2, R2))
+        if self._native_constraints:
+            out = tuple(q.contiguous() for q in out)
+            _lib.check(_lib.lib.wmx_mass_flux(self._plan, _lib.ptr_array(out), _lib.stream_ptr()))
+        for c in self._py_constraints:
+            out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
+        return out
 
     def max_dt(self, Q, time=0.0):
         """CFL time step (solver.py:135-154) as a 0-d device tensor."""
@@ -290,6 +366,8 @@ class Solver:
         """One SSP-RK3 step + constraints (solver.py:123-133,156-157)."""
         Q = self._as_state(Q)
         self._ensure_gauge(Q)
+        if self._fallback:
+            return self._advance_torch(Q, dt, float(time))
         out = tuple(torch.empty_like(q) for q in Q)
         if isinstance(dt, torch.Tensor):
             d = dt.reshape(-1).to(device=self._dev, dtype=torch.float64)
@@ -314,6 +392,9 @@ class Solver:
         two buffers).  Returns (Q_new, dt)."""
         Q = self._as_state(Q)
         self._ensure_gauge(Q)
+        if self._fallback:            # user BCs / sources: max_dt + the torch-driven stages
+            dt = self.max_dt(Q, time)
+            return self._advance_torch(Q, dt, float(time)), dt
         if not graph:                 # eager: the caller's stream, no side-stream handshake
             out = tuple(torch.empty_like(q) for q in Q)
             dt = torch.empty(16, dtype=torch.float64, device=self._dev)
