@@ -306,6 +306,15 @@ def bench_ensemble(args, wmx):
 
     el_s, Qseq = _timed_steps(seq_step, Qs0, args.steps, args.warmup)
     same = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qseq) for a, b in zip(Qa, Qb))
+    # the other launch structures of the same plan: captured-graph replay, and one
+    # persistent (cooperative) launch per RK stage
+    el_g, Qg = _timed_steps(lambda Qs: ens.step(Qs, graph=True)[0], Qs0, args.steps, args.warmup)
+    pers = wmx.EnsembleSolver(solvers)
+    pers.set_tiling(*best)
+    pers.set_persistent(True)
+    el_p, Qpe = _timed_steps(lambda Qs: pers.step(Qs)[0], Qs0, args.steps, args.warmup)
+    same_g = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qg) for a, b in zip(Qa, Qb))
+    same_p = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qpe) for a, b in zip(Qa, Qb))
     workload = f"ensemble of 16 uneven channel cases ({nblk} blocks) KEP2 WALE WM fp32"
     flux_ms, flux_n = prof["flux"]
     rec = _record(args, cells * args.steps / el_e / 1e6, el_e * 1e3 / args.steps, "f32-bulk+f64", workload,
@@ -316,6 +325,11 @@ def bench_ensemble(args, wmx):
     rec["sequential_value"] = round(cells * args.steps / el_s / 1e6, 2)
     rec["ensemble_speedup_vs_sequential"] = round(el_s / el_e, 3)
     rec["ensemble_equals_sequential"] = same
+    rec["launch_forms_ms_per_step"] = {"eager": round(el_e * 1e3 / args.steps, 4),
+                                       "graph_replay": round(el_g * 1e3 / args.steps, 4),
+                                       "persistent_stage": round(el_p * 1e3 / args.steps, 4)}
+    rec["graph_equals_eager"] = same_g
+    rec["persistent_equals_eager"] = same_p
     rec["state_finite"] = all(bool(torch.isfinite(q).all()) for Q in Qe for q in Q)
     rec["autotune_flux_ms"] = {k: round(v, 4) for k, v in timings.items()}
     return rec

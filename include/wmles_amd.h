@@ -242,6 +242,13 @@ int wmx_step(wmx_plan* plan, double* const* q0, double* const* qout, double* d_d
    count, the default); order 0 (block/i/j/k) or 1 (contiguous spans interleaved
    across the 8 XCDs, the default). */
 int wmx_plan_set_tiling(wmx_plan* plan, int32_t ci_len, int32_t order);
+/* FP32-bulk kernel form (no reference counterpart): on != 0 (the default) the flux kernel
+   evaluates the Green-Gauss gradients, SGS viscosity and Ducros sensor of each plane in
+   LDS as it marches, so no gradient records go through HBM and the separate gradient launch
+   keeps only its side work (per-case dt min, Newton wall faces); on == 0 runs the separate
+   gradient kernel and the flux kernel that reads its records.  Ignored by FP64 plans and
+   by persistent stages (which run the separate form). */
+int wmx_plan_set_fused_gradients(wmx_plan* plan, int32_t on);
 /* Persistent stages (the batching form's launch structure; no reference counterpart):
    on != 0 runs every RK stage of wmx_advance / wmx_step as ONE cooperative launch -- a
    grid of the kernel's co-resident maximum walks the stage's ghost-sweep phases, the

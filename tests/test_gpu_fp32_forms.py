@@ -183,3 +183,40 @@ def test_fp32_ramp_steps_close_to_fp64():
     et = abs(out["fp32"][2] - out["fp64"][2]) / abs(out["fp64"][2])
     print(f"ramp after 20 steps: mean u {eu:.2e}, tau_w {et:.2e}, t {out['fp64'][3]:.3e} s")
     assert eu <= 1e-4 and et <= 1e-4
+
+
+@pytest.mark.parametrize("kind", ["channel", "rot_cyc", "ramp"])
+def test_fused_gradients_match_the_separate_gradient_kernel(kind):
+    """The fused FP32-bulk kernel (gradient records evaluated in the flux march) against the
+    separate gradient kernel + flux kernel: the same FP32 arithmetic per gradient record, so
+    the RHS agrees to FP32 rounding of the records (bar 2e-6, max-normalised per component)
+    and one step to 1e-9 relative."""
+    if kind == "channel":
+        spec, arrays = cases.channel_spec(2, 48, 32, 16, {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}},
+                                          "vreman", Re_b=2e4, splits=[0, 20, 48])
+        pr = spec["props"]
+        prim = lambda xc: cases.channel_prim(xc, pr, 2)
+    elif kind == "rot_cyc":
+        spec, arrays = cases.multiblock_spec("rot_cyc", (12, 10, 8), {"kind": "Hybrid"},
+                                             {"kind": "WallModel", "T_wall": 300.0, "n_match": 1}, sgs="wale")
+        prim = cases.prim_fn
+    else:
+        spec, arrays, prim = cases.ramp_spec(wmx, 48, 24, 16, n_modes=8)
+    out = {}
+    for fused in (True, False):
+        mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": "fp32"})
+        s.set_fused_gradients(fused)
+        Q0 = s.init_state(prim)
+        Qn = cases.interior_noise(mesh.blocks, host(Q0), 11, 0.02)
+        R = host(s.rhs(Qn, 0.0))
+        Q1, dt = s.step(Qn, time=0.0)
+        out[fused] = (R, host(Q1), float(dt), mesh)
+    mesh = out[True][3]
+    assert out[True][2] == out[False][2]
+    for blk, a, b in zip(mesh.blocks, out[True][0], out[False][0]):
+        err = golden_io.rel_err(blk.interior(a), blk.interior(b))
+        print(kind, "fused vs separate RHS", err)
+        assert err <= 2e-6, err
+    for a, b in zip(out[True][1], out[False][1]):
+        err = float(np.abs(a - b).max() / np.abs(b).max())
+        assert err <= 1e-9, err

@@ -106,6 +106,7 @@ def _load():
         "wmx_step": ([P, PP, PP, P, C.c_double, C.c_int32, P], C.c_int),
         "wmx_plan_set_tiling": ([P, C.c_int32, C.c_int32], C.c_int),
         "wmx_plan_set_persistent": ([P, C.c_int32], C.c_int),
+        "wmx_plan_set_fused_gradients": ([P, C.c_int32], C.c_int),
         "wmx_tile_model": ([C.c_int32, P, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "wmx_plan_tiling": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "wmx_mass_flux": ([P, PP, P], C.c_int),

@@ -33,6 +33,7 @@ UNITS = {
     "rhs.hip": ["-ffp-contract=fast"],
     "flux.hip": ["-ffp-contract=fast"],
     "flux_ilp.hip": ["-ffp-contract=fast", "-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "flux3.hip": ["-ffp-contract=fast"],
     "mlp.hip": ["-ffp-contract=fast"],
     "ops.hip": ["-ffp-contract=fast"],
     "capi.hip": [],

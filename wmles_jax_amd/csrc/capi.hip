@@ -129,6 +129,7 @@ struct wmx_plan {
   double* d_delta = nullptr;
   // persistent stage launches (FP32-bulk plans without inflow, sponge, learned wall model
   // or caller-filled faces): one cooperative launch per RK stage
+  bool fuse_grad = false;             // FP32-bulk: gradient records evaluated in the flux march
   bool persistent = false;
   bool stage_dirty = true;            // d_stage needs re-uploading (tiling changed)
   StageStatic* d_stage = nullptr;
@@ -488,8 +489,11 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
                                p->d_case_off_all, p->d_part, nullptr, dt_out, s);
     }));
   }
+  // fused FP32-bulk plans: the flux kernel evaluates the gradient records itself; the
+  // gradient launch keeps only its head work (per-case dt min, Newton wall faces)
+  const bool grads = !(f32 && p->fuse_grad);
   WMX_CHECK(timed(p, K_GRAD, s, [&] {
-    return launch_grad(p->d_blks, p->d_pre_box, p->nb, nwg_of(p->pre_box), p->nd,
+    return launch_grad(p->d_blks, p->d_pre_box, p->nb, grads ? nwg_of(p->pre_box) : 0, p->nd,
                        wall_in_grad ? p->d_wall : nullptr, (int)p->wall.size(), p->wall_max, q, gdt, s);
   }));
   if (!wall_in_grad)
@@ -500,6 +504,14 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
         return launch_sponge_mean(p->d_blks, b, (long long)p->hb[b].n[0] * p->hb[b].n[1], q, s);
       }));
   return 0;
+}
+
+hipError_t plan_flux(const wmx_plan* p, int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles,
+                     int ci_len, const NumDev& nd, Ptrs qin, Ptrs q0, Ptrs out, const double* d_dt, double dt,
+                     hipStream_t s) {
+  if (nd.prec == WMX_PREC_FP32_BULK && p->fuse_grad)
+    return launch_flux3(mode, d_blks, d_tiles, ntiles, ci_len, nd, qin, q0, out, d_dt, dt, s);
+  return launch_flux(mode, d_blks, d_tiles, ntiles, ci_len, nd, qin, q0, out, d_dt, dt, s);
 }
 
 int check_plan(const wmx_plan* p) {
@@ -1080,7 +1092,7 @@ int wmx_rhs_filled(wmx_plan* p, double* const* q, double* const* r, void* stream
   hipStream_t s = as_stream(stream);
   Ptrs pq = make_ptrs(p, q), pr = make_ptrs(p, r);
   if (run_cells(p, pq, 0.0, nullptr, 0.0, s, nullptr, false, false)) return 1;
-  WMX_CHECK(launch_flux(0, p->d_blks, p->d_tiles, (int)p->tiles.size(), p->ci_len, p->nd, pq, pq, pr,
+  WMX_CHECK(plan_flux(p, 0, p->d_blks, p->d_tiles, (int)p->tiles.size(), p->ci_len, p->nd, pq, pq, pr,
                         nullptr, 0.0, s));
   WMX_CHECK(launch_shell_finish(0, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), pq, pr,
                                 nullptr, 0.0, s));
@@ -1092,7 +1104,7 @@ int wmx_rhs(wmx_plan* p, double* const* q, double* const* r, double time, void* 
   hipStream_t s = as_stream(stream);
   Ptrs pq = make_ptrs(p, q), pr = make_ptrs(p, r);
   if (run_cells(p, pq, time, nullptr, 0.0, s)) return 1;
-  WMX_CHECK(launch_flux(0, p->d_blks, p->d_tiles, (int)p->tiles.size(), p->ci_len, p->nd, pq, pq, pr,
+  WMX_CHECK(plan_flux(p, 0, p->d_blks, p->d_tiles, (int)p->tiles.size(), p->ci_len, p->nd, pq, pq, pr,
                         nullptr, 0.0, s));
   WMX_CHECK(launch_shell_finish(0, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), pq, pr,
                                 nullptr, 0.0, s));
@@ -1206,11 +1218,11 @@ int advance_impl(wmx_plan* p, double* const* q0, double* const* qout, double dt,
   // stage times t, t + dt, t + dt/2 (solver.py:125-129)
   const double t1 = d_dt ? time : time + dt, t2 = d_dt ? time : time + 0.5 * dt;
   if (run_cells(p, P0, time, nullptr, 0.0, s, dt_out, true)) return 1;
-  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return plan_flux(p, 1, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, P0, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t1, d_dt, 1.0, s, nullptr, true)) return 1;
-  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return plan_flux(p, 2, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
   if (run_cells(p, PO, t2, d_dt, 0.5, s, nullptr, true)) return 1;
-  WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_flux(3, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
+  WMX_CHECK(timed(p, K_FLUX, s, [&] { return plan_flux(p, 3, p->d_blks, p->d_tiles, ntiles, p->ci_len, p->nd, PO, P0, PO, d_dt, dt, s); }));
   WMX_CHECK(timed(p, K_SHELL, s, [&] {
     return launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s);
   }));
@@ -1324,6 +1336,13 @@ int wmx_plan_set_tiling(wmx_plan* p, int32_t ci_len, int32_t order) {
   return 0;
 }
 
+int wmx_plan_set_fused_gradients(wmx_plan* p, int32_t on) {
+  if (check_plan(p)) return 1;
+  p->fuse_grad = on != 0;
+  drop_graphs(p);
+  return 0;
+}
+
 int wmx_plan_set_persistent(wmx_plan* p, int32_t on) {
   if (check_plan(p)) return 1;
   if (!on) {
@@ -1357,7 +1376,7 @@ int wmx_step(wmx_plan* p, double* const* q0, double* const* qout, double* d_dt, 
     if (q0[b] == qout[b]) return fail("qout must not alias q0");
   // the step's dt comes from its stage-1 primitive launch; time-dependent inflow plans
   // run eagerly (a captured graph would hold one time)
-  if (!use_graph || has_inflow || s == nullptr)
+  if (!use_graph || has_inflow || s == nullptr || p->persistent)
     return advance_impl(p, q0, qout, 0.0, d_dt, d_dt, time, stream);
   for (auto& g : p->graphs) {
     bool same = g.d_dt == d_dt && g.time == time;

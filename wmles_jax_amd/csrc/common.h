@@ -144,6 +144,10 @@ hipError_t launch_wall(const BlockDev* d_blks, const WallOp* d_ops, int nops, in
 hipError_t launch_flux(int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles, int ci_len,
                        const NumDev& nd, Ptrs qin, Ptrs q0, Ptrs out, const double* d_dt,
                        double dt, hipStream_t s);
+// the fused FP32-bulk step kernel (flux3.h): gradient records evaluated in the march
+hipError_t launch_flux3(int mode, const BlockDev* d_blks, const int4* d_tiles, int ntiles, int ci_len,
+                        const NumDev& nd, Ptrs qin, Ptrs q0, Ptrs out, const double* d_dt, double dt,
+                        hipStream_t s);
 hipError_t launch_shell_save(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
                              hipStream_t s);
 hipError_t launch_shell_finish(int mode, const BlockDev* d_blks, const int* d_pre, int nblk, int nwg,

@@ -237,12 +237,13 @@ hipError_t launch_wm_eval(const NumDev& nd, int mode, long long n, const double*
 hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, const NumDev& nd,
                        const WallOp* d_wall, int nwall_ops, int wall_max, Ptrs q, const GradDt& gdt,
                        hipStream_t s) {
-  if (nwg <= 0) return hipSuccess;
   if (nd.prec == WMX_PREC_FP32_BULK) {
-    // side work in the same launch: per-case dt mins, then Newton wall faces (padded to 8)
+    // side work in the same launch: per-case dt mins, then Newton wall faces (padded to 8);
+    // nwg == 0 (fused plans): the side work alone
     const int wx = d_wall && nwall_ops > 0 && wall_max > 0 ? (wall_max + 255) / 256 : 0;
     const int ndt = gdt.out ? gdt.ncases : 0;
     const int head = ((ndt + wx * nwall_ops + 7) / 8) * 8;
+    if (nwg + head <= 0) return hipSuccess;
     const dim3 grid(nwg + head);
 #define WMX_GRAD32(S)                                                                          \
   hipLaunchKernelGGL(k_grad32<S>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, nd, head, d_wall, \
@@ -254,7 +255,7 @@ hipError_t launch_grad(const BlockDev* d_blks, const int* d_pre, int nblk, int n
       default: WMX_GRAD32(WMX_SGS_VREMAN);
     }
 #undef WMX_GRAD32
-  } else
+  } else if (nwg > 0)
     hipLaunchKernelGGL(k_grad_cell<double>, dim3(nwg), dim3(256), 0, s, d_blks, d_pre, nblk, nd);
   return hipGetLastError();
 }

@@ -424,6 +424,19 @@ class Solver:
             out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
         return out, dt
 
+    def set_fused_gradients(self, on=True):
+        """FP32-bulk plans: evaluate the gradient records inside the flux march (the
+        default) or in the separate gradient kernel."""
+        _lib.check(_lib.lib.wmx_plan_set_fused_gradients(self._plan, 1 if on else 0))
+
+    def set_persistent(self, on=True):
+        """Run every RK stage of advance / step as one cooperative (persistent) launch --
+        the sweep phases, primitives, gradients and flux tiles behind grid-wide barriers --
+        instead of one launch per kernel class (FP32-bulk plans without inflow, sponge,
+        learned wall model or user BCs; same results bit for bit)."""
+        _lib.check(_lib.lib.wmx_plan_set_persistent(self._plan, 1 if on else 0))
+        self.persistent = bool(on)
+
     def set_tiling(self, ci_len=0, order=1):
         """Flux-kernel tiling: i-planes per marching tile (0: automatic), XCD-interleaved
         order (1) or not (0)."""
