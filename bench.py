@@ -110,14 +110,18 @@ def profile_record(workload):
     return {}
 
 
-def run_form(wmx, form, size, steps, warmup, breakdown, graph=False):
-    """Time `steps` steps of one arithmetic form; returns timings, final state, wall stats."""
+def run_form(wmx, form, size, steps, warmup, breakdown, graph=False, fused=None):
+    """Time `steps` steps of one arithmetic form; returns timings, final state, wall stats.
+    fused: FP32-bulk kernel form (True: gradient records in the flux march; None: the
+    plan's default)."""
     import torch
     import cases
     nx, ny, nz = size
     spec, arrays = cases.channel_spec(1, nx, ny, nz, {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}},
                                       "vreman", T_wall=300.0, name=f"channel_{nx}x{ny}x{nz}")
     mesh, s = cases.build(wmx, spec, arrays, solver_kwargs={"precision": form})
+    if fused is not None and form == "fp32":
+        s.set_fused_gradients(fused)
     pr = spec["props"]
     Q = s.init_state(lambda xc: cases.channel_prim(xc, pr, 0))
     for _ in range(warmup):
@@ -194,7 +198,9 @@ def bench_channel(args, wmx):
     nx, ny, nz = args.size
     cells = nx * ny * nz
     forms = ["fp64", "fp32"] if args.form == "both" else [args.form]
-    res = {f: run_form(wmx, f, args.size, args.steps, args.warmup, args.breakdown, args.graph) for f in forms}
+    fused = {"auto": None, "on": True, "off": False}[args.fused]
+    res = {f: run_form(wmx, f, args.size, args.steps, args.warmup, args.breakdown, args.graph, fused)
+           for f in forms}
     head = "fp32" if "fp32" in res else "fp64"
     r = res[head]
     el = r["elapsed"]
@@ -207,7 +213,8 @@ def bench_channel(args, wmx):
                              "step": "Solver.step = max_dt + advance (SSP-RK3, 3 RHS, mass-flux "
                                      "constraint; the CFL min taken in the first stage's primitive pass)",
                              "graph": bool(args.graph),
-                             "form": head},
+                             "form": head,
+                             "fused_gradients": r["solver"]._fused if head == "fp32" else None},
                   _roofline(flux_ms / flux_n / 1e3, b_launch, workload, B_ALG_STEP[head]))
     rec["step_hbm_frac"] = round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
     ci, order, ntiles = r["solver"].tiling()
@@ -455,6 +462,8 @@ def main():
     ap.add_argument("--size", type=int, nargs=3, default=(256, 128, 128))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="time every kernel class (extra events)")
+    ap.add_argument("--fused", default="auto", choices=["auto", "on", "off"],
+                    help="FP32-bulk: gradient records in the flux march (on) or a separate kernel (off)")
     ap.add_argument("--graph", action="store_true",
                     help="replay captured hipGraphs for Solver.step (eager launches otherwise)")
     args = ap.parse_args()

@@ -129,14 +129,14 @@ struct wmx_plan {
   double* d_delta = nullptr;
   // persistent stage launches (FP32-bulk plans without inflow, sponge, learned wall model
   // or caller-filled faces): one cooperative launch per RK stage
-  bool fuse_grad = false;             // FP32-bulk: gradient records evaluated in the flux march
+  bool fuse_grad = true;              // FP32-bulk: gradient records evaluated in the flux march
   bool persistent = false;
   bool stage_dirty = true;            // d_stage needs re-uploading (tiling changed)
   StageStatic* d_stage = nullptr;
   Ptrs* d_pbuf = nullptr;
   int* d_fill_off = nullptr;
   int* d_fill_max = nullptr;
-  int stage_grid = 0;                 // persistent grid in use (0: not yet launched)
+  int stage_grid = 0, stage_grid_f = 0;   // persistent grids in use (0: not yet launched)
   // kernel-class event timing
   int prof_mask = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -468,14 +468,21 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
     sh.pre = p->d_sponge_pre;
     sh.nwg = p->sponge_pre.back();
   }
+  // fused FP32-bulk plans: the Newton wall faces ride in the primitive launch's head
+  const bool f32 = p->nd.prec == WMX_PREC_FP32_BULK;
+  const bool wall_in_prim = f32 && p->fuse_grad && !p->nd.net.w && !p->wall.empty() && p->wall_max > 0;
+  if (wall_in_prim) {
+    sh.wall = p->d_wall;
+    sh.wall_wgx = (p->wall_max + 255) / 256;
+    sh.nwall = (int)p->wall.size();
+  }
   WMX_CHECK(timed(p, K_PRIM, s, [&] {
     return launch_prim(p->d_blks, p->d_pre_all, p->nb, nwg_of(p->pre_all), q, p->nd,
                        dt_out ? p->d_part : nullptr, zfuse ? 1 : 0, sh, s);
   }));
-  // FP32-bulk plans take the per-case dt mins and (Newton model) the wall faces into the
-  // gradient launch; the flux kernel after it is the first reader of both
-  const bool f32 = p->nd.prec == WMX_PREC_FP32_BULK;
-  const bool wall_in_grad = f32 && !p->nd.net.w;
+  // FP32-bulk plans take the per-case dt mins and (Newton model, unfused) the wall faces into
+  // the gradient launch; the flux kernel after it is the first reader of both
+  const bool wall_in_grad = f32 && !p->nd.net.w && !wall_in_prim;
   GradDt gdt;
   if (dt_out && f32) {
     gdt.ncases = p->n_cases;
@@ -496,7 +503,7 @@ int run_cells(wmx_plan* p, Ptrs q, double time, const double* d_dt, double coef,
     return launch_grad(p->d_blks, p->d_pre_box, p->nb, grads ? nwg_of(p->pre_box) : 0, p->nd,
                        wall_in_grad ? p->d_wall : nullptr, (int)p->wall.size(), p->wall_max, q, gdt, s);
   }));
-  if (!wall_in_grad)
+  if (!wall_in_grad && !wall_in_prim)
     WMX_CHECK(timed(p, K_WALL, s, [&] { return launch_wall(p->d_blks, p->d_wall, (int)p->wall.size(), p->wall_max, q, p->nd, s); }));
   if (p->nd.n_sponge > 0 && !p->d_sponge_pre)
     for (int b = 0; b < p->nb; ++b)
@@ -1161,7 +1168,11 @@ int upload_stage(wmx_plan* p) {
   st.ngrad = nwg_of(p->pre_box);
   st.wall_ops = p->d_wall;
   st.nwall_ops = (int)p->wall.size();
-  st.wall_wgx = st.nwall_ops > 0 && p->wall_max > 0 ? (p->wall_max + 255) / 256 : 0;
+  const int wx = st.nwall_ops > 0 && p->wall_max > 0 ? (p->wall_max + 255) / 256 : 0;
+  st.grad_wall_wgx = p->fuse_grad ? 0 : wx;
+  st.prim_wall_wgx = p->fuse_grad ? wx : 0;
+  st.prim_head = p->fuse_grad ? (wx * st.nwall_ops + 7) / 8 * 8 : 0;
+  if (p->fuse_grad) st.ngrad = 0;                 // the gradient list keeps only its dt head
   st.dt_list = p->case_wg_all_ident ? nullptr : p->d_case_wg_all;
   st.dt_off = p->d_case_off_all;
   st.tiles = p->d_tiles;
@@ -1195,12 +1206,14 @@ int advance_persistent(wmx_plan* p, double* const* q0, double* const* qout, doub
     c.nfill = nph - (p->fuse_last ? 1 : 0);
     c.zfuse = p->fuse_last ? 1 : 0;
     c.ndt = mode == 1 && dt_out ? p->n_cases : 0;
-    c.head = ((c.ndt + wx * (int)p->wall.size()) + 7) / 8 * 8;
+    c.head = ((c.ndt + (p->fuse_grad ? 0 : wx * (int)p->wall.size())) + 7) / 8 * 8;
     c.part = c.ndt ? p->d_part : nullptr;
     c.dt_out = c.ndt ? dt_out : nullptr;
     c.d_dt = d_dt;
     c.dt_host = dt;
-    WMX_CHECK(timed(p, K_FLUX, s, [&] { return launch_stage(mode, sc, p->d_stage, c, &p->stage_grid, s); }));
+    WMX_CHECK(timed(p, K_FLUX, s, [&] {
+      return launch_stage(mode, sc, p->fuse_grad, p->d_stage, c, p->fuse_grad ? &p->stage_grid_f : &p->stage_grid, s);
+    }));
   }
   WMX_CHECK(launch_shell_finish(1, p->d_blks, p->d_pre_shell, p->nb, nwg_of(p->pre_shell), P0, PO, d_dt, dt, s));
   return wmx_mass_flux(p, qout, stream);
@@ -1339,6 +1352,7 @@ int wmx_plan_set_tiling(wmx_plan* p, int32_t ci_len, int32_t order) {
 int wmx_plan_set_fused_gradients(wmx_plan* p, int32_t on) {
   if (check_plan(p)) return 1;
   p->fuse_grad = on != 0;
+  p->stage_dirty = true;
   drop_graphs(p);
   return 0;
 }

@@ -117,9 +117,13 @@ hipError_t launch_geometry(const double* X, const int* N, double* vol, double* x
 // d_part != nullptr: the launch also writes the CFL partial of each workgroup (step fusion)
 // sponge row means taken by head workgroups of a primitive launch (pre == nullptr: none):
 // pre[b] = first head workgroup of block b (32 rows each), nwg = their total
+// FP32-bulk plans with fused gradients also give the Newton wall faces to head workgroups
+// of the primitive launch, after the sponge ones: wall_wgx workgroups per wall op.
 struct SpongeHead {
   const int* pre = nullptr;
   int nwg = 0;
+  const WallOp* wall = nullptr;
+  int wall_wgx = 0, nwall = 0;
   int head = 0;   // set by the launcher
 };
 // zfuse: the k ghosts flagged in BlockDev::zfuse are evaluated from the sweep (their fill
@@ -211,8 +215,9 @@ struct StageStatic {
   int nblk, nprim;
   const int* pre_box;          // gradient workgroups per block (prefix)
   int ngrad;
-  const WallOp* wall_ops;      // Newton wall faces in the gradient head
-  int wall_wgx, nwall_ops;
+  const WallOp* wall_ops;      // Newton wall faces: in the gradient head (grad_wall_wgx
+  int nwall_ops;               // workgroups per op) or, fused plans, in the primitive head
+  int grad_wall_wgx, prim_wall_wgx, prim_head;
   const int* dt_list;          // per-case workgroup lists of the CFL partials
   const int* dt_off;
   const int4* tiles;
@@ -227,10 +232,11 @@ struct StageCall {
   const double* d_dt;
   double dt_host;
 };
-// grid: workgroups of the persistent grid (0: the co-resident maximum, a multiple of 8,
-// returned in *grid_used)
-hipError_t launch_stage(int mode, int sc, const StageStatic* d_st, const StageCall& c, int* grid_used,
-                        hipStream_t s);
+// fused: the fused kernel form (gradient records in the flux march, wall faces in the
+// primitive head).  grid: workgroups of the persistent grid (0: the co-resident maximum, a
+// multiple of 8, returned in *grid_used)
+hipError_t launch_stage(int mode, int sc, bool fused, const StageStatic* d_st, const StageCall& c,
+                        int* grid_used, hipStream_t s);
 
 hipError_t launch_scheme_flux(const NumDev& nd, const int* n, int d, const double* w, const double* S,
                               const double* theta, double* F, hipStream_t s);

@@ -4,6 +4,7 @@
 // contraction, so the CFL partials equal k_dt_partial's bit for bit.
 #pragma once
 #include "ghostfill.h"
+#include "wallface.h"
 
 namespace wmx {
 
@@ -196,7 +197,16 @@ __device__ __forceinline__ void prim32_block(const BlockDev* __restrict__ blks,
                                              const Ptrs& q, const NumDev& nd, double* __restrict__ part,
                                              int zfuse, const SpongeHead& sh, int wgx, int nwg) {
   if (wgx < sh.head) {
-    sponge_head(blks, nblk, q, sh, wgx);
+    if (wgx < sh.nwg) {
+      sponge_head(blks, nblk, q, sh, wgx);
+    } else if (sh.wall_wgx > 0) {                 // Newton wall faces (reads interior q only)
+      const int h = wgx - sh.nwg, op_i = h / sh.wall_wgx;
+      if (op_i < sh.nwall) {
+        const WallOp op = sh.wall[op_i];
+        const int idx0 = (h - op_i * sh.wall_wgx) * (int)blockDim.x + (int)threadIdx.x;
+        if (idx0 < op.count) wall_face<false>(blks, op, idx0, q, nd, nullptr, nullptr);
+      }
+    }
     return;
   }
   // XCD-contiguous spans: the i faces one plane apart that the CFL partial reads twice

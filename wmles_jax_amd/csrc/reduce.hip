@@ -305,8 +305,9 @@ hipError_t launch_prim(const BlockDev* d_blks, const int* d_pre, int nblk, int n
                        const NumDev& nd, double* d_part, int zfuse, SpongeHead sh, hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
   const bool f32 = nd.prec == WMX_PREC_FP32_BULK;
-  sh.head = sh.pre ? ((sh.nwg + 7) / 8) * 8 : 0;
   if (!sh.pre) sh.nwg = 0;
+  if (!sh.wall) sh.wall_wgx = sh.nwall = 0;
+  sh.head = ((sh.nwg + sh.wall_wgx * sh.nwall) + 7) / 8 * 8;   // (keeps the XCD spans)
   const dim3 grid(nwg + sh.head);
   if (d_part) {
     if (f32) hipLaunchKernelGGL(k_prim32<true>, grid, dim3(256), 0, s, d_blks, d_pre, nblk, q, nd, d_part, zfuse, sh);

@@ -25,8 +25,12 @@ __device__ void st_grad(const StageStatic& s, int head, int ndt, const double* _
                         double* __restrict__ dt_out, int wgx);
 template <int MODE, int SC>
 __device__ void st_flux(const StageStatic& s, const double* __restrict__ d_dt, double dt_host, int tile);
-
 template <int MODE, int SC>
+__device__ void st_flux3(const StageStatic& s, const double* __restrict__ d_dt, double dt_host, int tile);
+
+// FUSED: the fused kernel form (flux3 tiles with the gradient records in the march, wall
+// faces in the primitive head, the gradient list only its dt head)
+template <int MODE, int SC, bool FUSED>
 __global__ __launch_bounds__(256, 2) void k_stage(const StageStatic* __restrict__ st, StageCall c) {
   cg::grid_group grid = cg::this_grid();
   const StageStatic& s = *st;
@@ -46,7 +50,8 @@ __global__ __launch_bounds__(256, 2) void k_stage(const StageStatic* __restrict_
     for (int w = (int)blockIdx.x; w < nwork; w += G) st_fill(s, o + w / per, (w % per) * 256 + (int)threadIdx.x);
     grid.sync();
   }
-  for (int w = (int)blockIdx.x; w < s.nprim; w += G) {
+  const int npr = s.prim_head + s.nprim;
+  for (int w = (int)blockIdx.x; w < npr; w += G) {
     if (c.part) st_prim<true>(s, c.part, c.zfuse, w);          // (uniform: a kernel argument)
     else st_prim<false>(s, c.part, c.zfuse, w);
     __syncthreads();                                            // LDS reuse by the next item
@@ -59,15 +64,16 @@ __global__ __launch_bounds__(256, 2) void k_stage(const StageStatic* __restrict_
   }
   grid.sync();
   for (int w = (int)blockIdx.x; w < s.ntiles; w += G) {
-    st_flux<MODE, SC>(s, c.d_dt, c.dt_host, w);
+    if (FUSED) st_flux3<MODE, SC>(s, c.d_dt, c.dt_host, w);
+    else st_flux<MODE, SC>(s, c.d_dt, c.dt_host, w);
     __syncthreads();
   }
 }
 
 namespace {
-template <int MODE, int SC>
+template <int MODE, int SC, bool FUSED>
 hipError_t coop(const StageStatic* d_st, const StageCall& c, int grid, int* used, hipStream_t s) {
-  const void* fn = (const void*)k_stage<MODE, SC>;
+  const void* fn = (const void*)k_stage<MODE, SC, FUSED>;
   if (grid <= 0) {
     int per_cu = 0, dev = 0, cus = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0);
@@ -85,14 +91,16 @@ hipError_t coop(const StageStatic* d_st, const StageCall& c, int grid, int* used
 }
 }  // namespace
 
-hipError_t launch_stage(int mode, int sc, const StageStatic* d_st, const StageCall& c, int* grid_used,
-                        hipStream_t s) {
+hipError_t launch_stage(int mode, int sc, bool fused, const StageStatic* d_st, const StageCall& c,
+                        int* grid_used, hipStream_t s) {
   const int grid = grid_used && *grid_used > 0 ? *grid_used : 0;
-#define WMX_STAGE_M(C)                                                \
-  switch (mode) {                                                     \
-    case 1: return coop<1, C>(d_st, c, grid, grid_used, s);           \
-    case 2: return coop<2, C>(d_st, c, grid, grid_used, s);           \
-    default: return coop<3, C>(d_st, c, grid, grid_used, s);          \
+#define WMX_STAGE_F(M, C) \
+  return fused ? coop<M, C, true>(d_st, c, grid, grid_used, s) : coop<M, C, false>(d_st, c, grid, grid_used, s);
+#define WMX_STAGE_M(C)                 \
+  switch (mode) {                      \
+    case 1: WMX_STAGE_F(1, C)          \
+    case 2: WMX_STAGE_F(2, C)          \
+    default: WMX_STAGE_F(3, C)         \
   }
   switch (sc) {
     case SC_HYB_KEP4_ALBADA: WMX_STAGE_M(SC_HYB_KEP4_ALBADA)
@@ -101,6 +109,7 @@ hipError_t launch_stage(int mode, int sc, const StageStatic* d_st, const StageCa
     default: WMX_STAGE_M(SC_ANY)
   }
 #undef WMX_STAGE_M
+#undef WMX_STAGE_F
 }
 
 }  // namespace wmx

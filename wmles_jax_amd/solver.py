@@ -21,6 +21,10 @@ from .schemes import KEP, KEEP, MUSCLHLLC, Hybrid
 from .sgs import SGSModel, make_sgs
 
 
+# FP32-bulk plans evaluate the gradient records inside the flux march unless set otherwise
+FUSED_GRADIENTS_DEFAULT = True
+
+
 class ConstantMassFlux:
     """Post-step constraint: shift the volume-averaged rho u_dir of all block
     interiors to `target`, keeping the internal energy (solver.py:16-36).  Inside
@@ -176,6 +180,9 @@ class Solver:
         _lib.check(_lib.lib.wmx_plan_wall_faces(plan, C.byref(cnt), rec))
         self._wall_faces = [tuple(rec[5 * i:5 * i + 5]) for i in range(cnt.value)]
         self._dt_buf = torch.empty(16, dtype=torch.float64, device=self._dev)
+        self._fused = precision == "fp32" and FUSED_GRADIENTS_DEFAULT
+        if precision == "fp32":
+            self.set_fused_gradients(self._fused)
         # user BCs (no HIP descriptor) and user sources run on the torch side: the sweep
         # is then driven phase by phase from here, and the RK3 stages combine in torch
         from .smooth_ramp import SmoothRampSponge
@@ -428,6 +435,7 @@ class Solver:
         """FP32-bulk plans: evaluate the gradient records inside the flux march (the
         default) or in the separate gradient kernel."""
         _lib.check(_lib.lib.wmx_plan_set_fused_gradients(self._plan, 1 if on else 0))
+        self._fused = bool(on)
 
     def set_persistent(self, on=True):
         """Run every RK stage of advance / step as one cooperative (persistent) launch --
