@@ -93,7 +93,7 @@ def cpu_baseline(sample=(32, 128, 128), steps=4):
                       f"(element-wise, one thread), {el:.1f} s on the host"}
 
 
-def profile_record(workload):
+def profile_record(workload, fused=True):
     """The committed rocprofv3 figures of the flux kernel for `workload` (HBM bytes per
     launch from FETCH_SIZE / WRITE_SIZE, average launch time from the kernel trace), or
     {} when none matches."""
@@ -105,7 +105,7 @@ def profile_record(workload):
     except ValueError:
         return {}
     for rec in recs if isinstance(recs, list) else [recs]:
-        if rec.get("workload") == workload:
+        if rec.get("workload") == workload and rec.get("fused_gradients", True) == fused:
             return rec
     return {}
 
@@ -165,15 +165,20 @@ def form_errors(r64, r32):
             "q_w": float(abs(qw(st32) - qw(st64)) / abs(qw(st64)))}
 
 
-def _roofline(avg_launch_s, b_launch, workload, b_cell_step):
+FLUX_KERNEL = {True: "k_flux3 (gradient records in the march + face fluxes + divergence + RK stage)",
+               False: "k_flux2 (face fluxes + divergence + RK stage; gradient records from k_grad32)",
+               None: "k_flux1 (FP64: face fluxes + divergence + RK stage)"}
+
+
+def _roofline(avg_launch_s, b_launch, workload, b_cell_step, fused=True):
     """achieved = algorithmic bytes per flux launch / the launch time measured live with
     HIP events on the launch stream.  The committed rocprofv3 trace's average launch
     time for the same workload, and the fraction it gives, are reported beside it."""
     achieved = b_launch / avg_launch_s / 1e9
-    prof = profile_record(workload)
+    prof = profile_record(workload, fused)
     rec = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": prof.get("hbm_bytes_per_launch"),
-           "kernel": "k_flux (fused face fluxes + divergence + RK stage)",
+           "kernel": FLUX_KERNEL[fused],
            "alg_bytes_per_cell_step": b_cell_step,
            "alg_bytes_per_launch": round(b_launch), "avg_launch_ms": round(avg_launch_s * 1e3, 4)}
     us = prof.get("rocprof_avg_launch_us")
@@ -182,6 +187,9 @@ def _roofline(avg_launch_s, b_launch, workload, b_cell_step):
         rec["rocprof_frac"] = round(b_launch / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
     if rec["traffic"]:
         rec["traffic_over_alg"] = round(rec["traffic"] / b_launch, 3)
+    for k in ("valu_issue_busy", "valu_lane_utilisation"):
+        if k in prof:
+            rec["rocprof_" + k] = prof[k]
     return rec
 
 
@@ -215,7 +223,8 @@ def bench_channel(args, wmx):
                              "graph": bool(args.graph),
                              "form": head,
                              "fused_gradients": r["solver"]._fused if head == "fp32" else None},
-                  _roofline(flux_ms / flux_n / 1e3, b_launch, workload, B_ALG_STEP[head]))
+                  _roofline(flux_ms / flux_n / 1e3, b_launch, workload, B_ALG_STEP[head],
+                            r["solver"]._fused if head == "fp32" else None))
     rec["step_hbm_frac"] = round(B_ALG_STEP[head] * cells / (el / args.steps) / 1e9 / HBM_PEAK_GBS, 4)
     ci, order, ntiles = r["solver"].tiling()
     rec["config"]["tiling"] = {"ci_len": ci, "xcd_order": order, "tiles": ntiles}
