@@ -120,7 +120,7 @@ struct wmx_plan {
   std::vector<InflowDev> inflows;
   // workspace
   size_t ws_bytes = 0;
-  std::vector<size_t> off_P, off_G, off_shell, off_sponge, off_geo32;
+  std::vector<size_t> off_P, off_G, off_shell, off_sponge, off_geo32, off_sb;
   std::vector<std::vector<long long>> off_wall;   // per block, 6 faces (-1: none)
   size_t off_part = 0, off_scal = 0, off_stat = 0;
   double* d_stat = nullptr;           // plane-statistics partials
@@ -761,6 +761,8 @@ int wmx_plan_create(int32_t nblocks, const wmx_block_desc* blocks, const wmx_num
     if (nd.prec == WMX_PREC_FP32_BULK)
       off = align_up(off + sizeof(float) * (3 * (size_t)(fcount(B.N, 0) + fcount(B.N, 1) + fcount(B.N, 2)) +
                                             (size_t)B.cells));
+    p->off_sb.push_back(off);
+    off = align_up(off + sizeof(double) * NSB_PLANES * (size_t)B.cells);
     p->off_shell.push_back(off);
     off = align_up(off + sizeof(double) * 5 * B.shell_cells);
     p->off_sponge.push_back(off);
@@ -869,6 +871,7 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
       B.Sf[0] = B.Sf[1] = B.Sf[2] = nullptr;
       B.volf = nullptr;
     }
+    B.Sb = reinterpret_cast<double*>(p->ws + p->off_sb[b]);
     B.shell = reinterpret_cast<double*>(p->ws + p->off_shell[b]);
     B.sponge_mean = reinterpret_cast<double*>(p->ws + p->off_sponge[b]);
     B.sponge_row = p->nd.n_sponge > 0
@@ -881,6 +884,11 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
   p->d_delta = reinterpret_cast<double*>(p->ws + p->off_scal);
   p->d_stat = reinterpret_cast<double*>(p->ws + p->off_stat);
   WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
+  // the CFL dt's cell-mean face vectors, made once: the geometry may still be in flight on
+  // the caller's stream, so the device is synchronised on both sides
+  WMX_CHECK(hipDeviceSynchronize());
+  for (int b = 0; b < p->nb; ++b) WMX_CHECK(launch_cell_sb(p->hb[b], nullptr));
+  WMX_CHECK(hipDeviceSynchronize());
   if (p->nd.n_sponge > 0) {
     // rows of each block that need the sponge mean (static geometry; made once)
     WMX_CHECK(hipDeviceSynchronize());

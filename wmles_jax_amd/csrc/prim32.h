@@ -50,29 +50,49 @@ __device__ __forceinline__ unsigned interior_cell(const BlockDev& B, unsigned l,
   return ((unsigned)*I * (unsigned)B.N[1] + (unsigned)*J) * (unsigned)B.N[2] + (unsigned)*K;
 }
 
-// CFL time step of interior cell (I, J, K) (ghosted index c) from its primitives w;
-// solver.py:139-150.  Every load first (one memory round trip), then the arithmetic.  The
-// dt kernel and the fused stage-1 primitive kernels below share it in this unit, so the
-// two give the same bits.
-__device__ __forceinline__ double cell_dt(const BlockDev& B, const NumDev& nd, unsigned c, int I,
-                                          int J, int K, const Prim& w) {
-  double Sv[3][3][2];
-  const double vol = ldu(B.vol, c);
+// CFL metric of cell (I, J, K): the cell means of its face vectors
+// Sb[d] = 0.5 (S_lo + S_hi) in m[3 d + x] and their norms |Sb[d]| in m[9 + d]
+// (solver.py:145-146).  The per-cell geometry part of the CFL dt, which k_cell_sb stores at
+// workspace binding; cell_dt evaluates it itself when the plan has no such store (the same
+// arithmetic in the same unit, so the same bits).
+constexpr int NSB = 12;
+static_assert(NSB == NSB_PLANES, "workspace planes of the CFL metric");
+__device__ __forceinline__ void cell_metric(const BlockDev& B, int I, int J, int K, double* m) {
   const int cc[3] = {I, J, K};
+  double lo[3][3], hi[3][3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double* S = B.S[d];
     const unsigned fc = (unsigned)fcount(B.N, d);
-    int hi[3] = {cc[0], cc[1], cc[2]};
-    hi[d] += 1;
+    int up[3] = {cc[0], cc[1], cc[2]};
+    up[d] += 1;
     const unsigned f0 = (unsigned)fidx(B.N, d, cc[0], cc[1], cc[2]);
-    const unsigned f1 = (unsigned)fidx(B.N, d, hi[0], hi[1], hi[2]);
+    const unsigned f1 = (unsigned)fidx(B.N, d, up[0], up[1], up[2]);
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      Sv[d][x][0] = ldu(S + x * fc, f0);
-      Sv[d][x][1] = ldu(S + x * fc, f1);
+      lo[d][x] = ldu(S + x * fc, f0);
+      hi[d][x] = ldu(S + x * fc, f1);
     }
   }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) m[3 * d + x] = 0.5 * (lo[d][x] + hi[d][x]);
+    m[9 + d] = sqrt((m[3 * d] * m[3 * d] + m[3 * d + 1] * m[3 * d + 1]) + m[3 * d + 2] * m[3 * d + 2]);
+  }
+}
+
+// cell_dt's operands from the stored metric (planes B.Sb) and the volume of cell c
+__device__ __forceinline__ void cell_dt_load(const BlockDev& B, unsigned c, double* m, double* vol) {
+  const size_t cells = (size_t)B.cells;
+  *vol = ldu(B.vol, c);
+#pragma unroll
+  for (int i = 0; i < NSB; ++i) m[i] = ldu(B.Sb + i * cells, c);
+}
+
+// CFL time step of a cell from its primitives w, metric m and volume; solver.py:139-150, in
+// the reference's association order (bit-exact)
+__device__ __forceinline__ double cell_dt_eval(const NumDev& nd, const Prim& w, const double* m, double vol) {
   const GasC& gas = nd.gas;
   const double cs = sqrt(gas.gamma * gas.R * w.T);
   const double cnu = fmax(4.0 / 3.0, gas.gamma / gas.Pr);
@@ -80,14 +100,26 @@ __device__ __forceinline__ double cell_dt(const BlockDev& B, const NumDev& nd, u
   double lam = 0.0, lamv = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double s0 = 0.5 * (Sv[d][0][0] + Sv[d][0][1]);
-    const double s1 = 0.5 * (Sv[d][1][0] + Sv[d][1][1]);
-    const double s2 = 0.5 * (Sv[d][2][0] + Sv[d][2][1]);
-    const double Sm = sqrt((s0 * s0 + s1 * s1) + s2 * s2);
-    lam = (lam + fabs(dot3(w.u, w.v, w.w, s0, s1, s2))) + cs * Sm;
+    const double Sm = m[9 + d];
+    lam = (lam + fabs(dot3(w.u, w.v, w.w, m[3 * d], m[3 * d + 1], m[3 * d + 2]))) + cs * Sm;
     lamv = lamv + (nu * (Sm * Sm)) / vol;
   }
   return nd.cfl / ((lam + 4.0 * lamv) / vol);
+}
+
+// CFL time step of interior cell (I, J, K) (ghosted index c) from its primitives w.  Every
+// load first (one memory round trip), then the arithmetic.  The dt kernel and the fused
+// stage-1 primitive kernels below share it in this unit, so the two give the same bits.
+__device__ __forceinline__ double cell_dt(const BlockDev& B, const NumDev& nd, unsigned c, int I,
+                                          int J, int K, const Prim& w) {
+  double m[NSB], vol;
+  if (B.Sb) {                                    // stored at workspace binding (k_cell_sb)
+    cell_dt_load(B, c, m, &vol);
+  } else {
+    vol = ldu(B.vol, c);
+    cell_metric(B, I, J, K, m);
+  }
+  return cell_dt_eval(nd, w, m, vol);
 }
 
 // Spanwise mean of sponge row `row` for rows of 9..129 cells (numpy's pairwise order for

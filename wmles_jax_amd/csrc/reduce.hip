@@ -294,6 +294,28 @@ __global__ __launch_bounds__(256) void k_sponge_mean(const BlockDev* __restrict_
 }
 
 // ----------------------------------------------------------------------- launchers
+// CFL metric of every cell of a block (cell_metric: the cell means of the face vectors and
+// their norms; the face arrays hold N + 1 faces along each direction), made once at
+// workspace binding.
+__global__ __launch_bounds__(256) void k_cell_sb(BlockDev B) {
+  const unsigned cells = (unsigned)B.cells;
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const unsigned N1 = (unsigned)B.N[1], N2 = (unsigned)B.N[2];
+  const int K = (int)(c % N2), J = (int)((c / N2) % N1), I = (int)(c / (N1 * N2));
+  double m[NSB];
+  cell_metric(B, I, J, K, m);
+#pragma unroll
+  for (int i = 0; i < NSB; ++i) B.Sb[(size_t)i * cells + c] = m[i];
+}
+
+hipError_t launch_cell_sb(const BlockDev& hb, hipStream_t s) {
+  if (!hb.Sb) return hipSuccess;
+  const long long nb = (hb.cells + 255) / 256;
+  hipLaunchKernelGGL(k_cell_sb, dim3((unsigned)nb), dim3(256), 0, s, hb);
+  return hipGetLastError();
+}
+
 hipError_t launch_dt_partial(const BlockDev* d_blks, const int* d_pre, int nblk, int nwg, Ptrs q,
                              const NumDev& nd, double* d_part, hipStream_t s) {
   if (nwg <= 0) return hipSuccess;
