@@ -114,7 +114,7 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
                                               const CellWT<float>& L, const CellWT<float>& Rc,
                                               const CellWT<float>& R2, float thL, float thR,
                                               const CellGT<float>& gL, const CellGT<float>& gR,
-                                              float TL, float TR, const double* xL, const double* xR,
+                                              float TL, float TR, float d0, float d1, float d2,
                                               float sx, float sy, float sz, float pr, float er,
                                               float* Ff, float* uw) {
   const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
@@ -142,7 +142,6 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
     Ff[4] = Ff[4] + s * (Fu[4] - er * Fu[0]);
   }
   *uw = wc * un4;
-  const float d0 = float(xR[0] - xL[0]), d1 = float(xR[1] - xL[1]), d2 = float(xR[2] - xL[2]);
   const float dist = fsqrt<float>((d0 * d0 + d1 * d1) + d2 * d2);
   const float id = fdiv<float>(1.f, dist);
   const float phL[4] = {L.u, L.v, L.w, TL};
@@ -418,23 +417,21 @@ __device__ __forceinline__ void flux2_tile(const BlockDev* __restrict__ blks,
       CellWT<R> L2, L, Rc, R2;
       CellGT<R> gL, gR;
       R thL, thR, TL, TR;
-      double xL[3], xR[3];
+      float dx[3] = {0.f, 0.f, 0.f};           // centroid difference, FP64 then rounded
       if (task == 2) {
         fI = I + 1;
         L2 = Pw[0].w; L = Pw[1].w; Rc = Pw[2].w; R2 = Pw[3].w;
         TL = Pw[1].T; TR = Pw[2].T;
         gR = Gw[1].g; thR = Gw[1].th;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) xR[x] = Xw[1][x];
         if (first) {
           gL = Gw[0].g; thL = Gw[0].th;
 #pragma unroll
-          for (int x = 0; x < 3; ++x) xL[x] = Xw[0][x];
+          for (int x = 0; x < 3; ++x) dx[x] = float(Xw[1][x] - Xw[0][x]);
         } else {                                     // plane I of the own column, staged
           const int sg = (tj + 1) * G2P + (tk + 1);
           gL = sg_g<R>(sG, sg); thL = sG[14][sg];
 #pragma unroll
-          for (int x = 0; x < 3; ++x) xL[x] = sX[x][sg];
+          for (int x = 0; x < 3; ++x) dx[x] = float(Xw[1][x] - sX[x][sg]);
         }
       } else {
         if (task == 0) {
@@ -460,7 +457,7 @@ __device__ __forceinline__ void flux2_tile(const BlockDev* __restrict__ blks,
           gL = sg_g<R>(sG, gR_ - gst); gR = sg_g<R>(sG, gR_);
           thL = sG[14][gR_ - gst]; thR = sG[14][gR_];
 #pragma unroll
-          for (int x = 0; x < 3; ++x) { xL[x] = sX[x][gR_ - gst]; xR[x] = sX[x][gR_]; }
+          for (int x = 0; x < 3; ++x) dx[x] = float(sX[x][gR_] - sX[x][gR_ - gst]);
         }
       }
       float F[NFX] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -481,7 +478,7 @@ __device__ __forceinline__ void flux2_tile(const BlockDev* __restrict__ blks,
         } else {
           const unsigned fi = (unsigned)fidx(N, d, fI, fJ, fK), fc = (unsigned)fcount(N, d);
           const float* S = B.Sf[d];
-          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
+          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, dx[0], dx[1], dx[2], ldu(S, fi),
                             ldu(S + fc, fi), ldu(S + 2 * fc, fi), float(B.p_ref), float(B.e_ref), F,
                             &F[5]);
         }

@@ -48,7 +48,8 @@ __device__ __forceinline__ CellWT<float> rec_w(const Rec32& a) {
 // staged planes: the per-cell arithmetic of k_grad32 (grad32.h), faces in the order i, j, k.
 __device__ __forceinline__ void grad_cell3(const BlockDev& B, const NumDev& nd, const float (*pm)[NP2],
                                            const float (*p0)[NP2], const float (*pp)[NP2],
-                                           const float (*pr)[NP2], int ps, int I, int J, int K, float* G) {
+                                           const float (*pr)[NP2], int ps, int I, int J, int K, float* G,
+                                           float* Slo) {
   const int N1 = B.N[1], N2 = B.N[2];
   const unsigned st0 = (unsigned)N1 * N2;
   const unsigned fc0 = (unsigned)fcount(B.N, 0), fc1 = (unsigned)fcount(B.N, 1), fc2 = (unsigned)fcount(B.N, 2);
@@ -118,6 +119,12 @@ __device__ __forceinline__ void grad_cell3(const BlockDev& B, const NumDev& nd, 
   G[12] = mu + mut;
   G[13] = (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t));
   G[14] = ducros<float>(g);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    Slo[x] = S0lo[x];
+    Slo[3 + x] = S1lo[x];
+    Slo[6 + x] = S2lo[x];
+  }
 }
 
 }  // namespace
@@ -134,8 +141,9 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
   __shared__ float sPr[4][NP2];                  // (rho, p', mu, e') of the current plane
   __shared__ float sG[NGW][NG2S];                // gradient records of the current plane
   __shared__ double sX[3][NG2S];                 // centroids of the current plane
-  __shared__ float sFj[2][NFX][NT];              // slot tile_tid(r, c), planes alternating
-  __shared__ float sFk[2][NFX][NT + C2J + 1];    // slot kface_slot(r, c)
+  __shared__ float sS[3][3][NG2S];               // lower i / j / k face vectors of the gradient cells
+  __shared__ float sFj[NFX][NT];                 // slot tile_tid(r, c)
+  __shared__ float sFk[NFX][NT + C2J + 1];       // slot kface_slot(r, c)
   const int4 t = tiles[tile];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
@@ -231,9 +239,11 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
   }
   ring_load(I0, I0 - 1);
   float Gp[NGW];                                 // gradient record of the own cell at I-1
-  float Fih[NFX];                                // own i face I-3/2 (lower face of cell I-1)
+  // partial outward sum of the own cell I-1 in FP64: its lower i face minus its j / k face
+  // sums (from the plane's exchange); the upper i face is subtracted when it is evaluated
+  double Cp[NFX];
 #pragma unroll
-  for (int v = 0; v < NFX; ++v) Fih[v] = 0.f;
+  for (int v = 0; v < NFX; ++v) Cp[v] = 0.0;
 #pragma unroll
   for (int x = 0; x < 3; ++x) Xp[x] = 0.0;
 #pragma unroll
@@ -243,7 +253,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
     // ---- window shift and staging of (u v w T') at I+1, (r p' mu e') and centroids at I
     Pw[0] = Pw[1]; Pw[1] = Pw[2]; Pw[2] = Pw[3]; Pw[3] = Pn;
     const int s_m = slot(I - 1), s_0 = slot(I), s_p = slot(I + 1);
-    const int fb = I & 1, fb1 = fb ^ 1;            // exchange halves of planes I and I-1
     if (own_p) {
       sPh[s_p][0][sp_own] = Pw[3].u; sPh[s_p][1][sp_own] = Pw[3].v;
       sPh[s_p][2][sp_own] = Pw[3].w; sPh[s_p][3][sp_own] = Pw[3].T;
@@ -278,11 +287,13 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         const int gj = r / G2K, gk = r - gj * G2K;
         const int JJ = ng + j0 - 1 + gj, KK = ng + k0 - 1 + gk;
         if (!g_ok(JJ, KK)) continue;
-        float Gv[NGW];
-        grad_cell3(B, nd, sPh[s_m], sPh[s_0], sPh[s_p], sPr, (gj + 1) * H2K + (gk + 1), I, JJ, KK, Gv);
+        float Gv[NGW], Slo[9];
+        grad_cell3(B, nd, sPh[s_m], sPh[s_0], sPh[s_p], sPr, (gj + 1) * H2K + (gk + 1), I, JJ, KK, Gv, Slo);
         const int gsl = gj * G2P + gk;
 #pragma unroll
         for (int v = 0; v < NGW; ++v) sG[v][gsl] = Gv[v];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) sS[v / 3][v % 3][gsl] = Slo[v];
       }
     }
     __syncthreads();
@@ -297,7 +308,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       CellWT<float> L2, L, Rc, R2;
       CellGT<float> gL, gR;
       float thL, thR, TL, TR;
-      double xL[3], xR[3];
+      float dx[3] = {0.f, 0.f, 0.f};           // centroid difference, FP64 then rounded
       if (task == 2) {
         L2 = rec_w(Pw[0]); L = rec_w(Pw[1]); Rc = rec_w(Pw[2]); R2 = rec_w(Pw[3]);
         TL = Pw[1].T; TR = Pw[2].T;
@@ -306,7 +317,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         gL.mu = Gp[12]; gL.k = Gp[13]; thL = Gp[14];
         gR = sg_g<float>(sG, sg_own); thR = sG[14][sg_own];
 #pragma unroll
-        for (int x = 0; x < 3; ++x) { xL[x] = Xp[x]; xR[x] = sX[x][sg_own]; }
+        for (int x = 0; x < 3; ++x) dx[x] = float(sX[x][sg_own] - Xp[x]);
       } else {
         if (task == 0) {
           d = 1;
@@ -336,7 +347,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
           gL = sg_g<float>(sG, gR_ - gst); gR = sg_g<float>(sG, gR_);
           thL = sG[14][gR_ - gst]; thR = sG[14][gR_];
 #pragma unroll
-          for (int x = 0; x < 3; ++x) { xL[x] = sX[x][gR_ - gst]; xR[x] = sX[x][gR_]; }
+          for (int x = 0; x < 3; ++x) dx[x] = float(sX[x][gR_] - sX[x][gR_ - gst]);
         }
       }
       float F[NFX] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -355,21 +366,20 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
 #pragma unroll
           for (int v = 0; v < 5; ++v) F[v] = (float)ldu(W + v * nw, w);
         } else {
-          const unsigned fi = (unsigned)fidx(N, d, fI, fJ, fK), fc = (unsigned)fcount(N, d);
-          const float* S = B.Sf[d];
-          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, xL, xR, ldu(S, fi),
-                            ldu(S + fc, fi), ldu(S + 2 * fc, fi), float(B.p_ref), float(B.e_ref), F,
-                            &F[5]);
+          // the face vector, staged by the gradient cell whose lower face it is
+          const int ss = task == 2 ? sg_own : (rj + 1) * G2P + (rk + 1);
+          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, dx[0], dx[1], dx[2], sS[d][0][ss], sS[d][1][ss],
+                            sS[d][2][ss], float(B.p_ref), float(B.e_ref), F, &F[5]);
         }
         if (task == 2) {
 #pragma unroll
           for (int v = 0; v < NFX; ++v) Fin[v] = F[v];
         } else if (d == 1) {
 #pragma unroll
-          for (int v = 0; v < NFX; ++v) sFj[fb][v][tile_tid(rj, rk)] = F[v];
+          for (int v = 0; v < NFX; ++v) sFj[v][tile_tid(rj, rk)] = F[v];
         } else {
 #pragma unroll
-          for (int v = 0; v < NFX; ++v) sFk[fb][v][kface_slot(rj, rk)] = F[v];
+          for (int v = 0; v < NFX; ++v) sFk[v][kface_slot(rj, rk)] = F[v];
         }
       }
     }
@@ -386,17 +396,11 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
     if (inner) __syncthreads();                  // (uniform) j / k face exchange
     if (fin) {
       ivol = 1.0 / ivol;
-      // outward sums in FP64 of the FP32 face values (j / k faces of plane I-1 from the
-      // exchange half it wrote); the energy gets the gauge terms e_ref x (mass divergence) +
-      // p_ref x (sum of w)
-      const int jup = tile_tid(tj + 1, tk), kup = kface_slot(tj, tk + 1);
+      // outward sums in FP64 of the FP32 face values; the energy gets the gauge terms
+      // e_ref x (mass divergence) + p_ref x (sum of w)
       double a[NFX];
 #pragma unroll
-      for (int v = 0; v < NFX; ++v) {
-        double t = 0.0 - ((double)Fin[v] - (double)Fih[v]);
-        t = t - ((double)sFj[fb1][v][jup] - (double)sFj[fb1][v][tid]);
-        a[v] = t - ((double)sFk[fb1][v][kup] - (double)sFk[fb1][v][tid]);
-      }
+      for (int v = 0; v < NFX; ++v) a[v] = Cp[v] - (double)Fin[v];
       a[4] = (a[4] + B.e_ref * a[0]) + B.p_ref * a[5];
       double r[5];
 #pragma unroll
@@ -433,9 +437,15 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         for (int v = 0; v < 5; ++v) stu(o + v * cells, c, a0 * vz[v] + a1 * (vq[v] + dt * r[v]));
       }
     }
-    // ---- carry: the lower i face, gradient record and centroid of the own cell I
+    // ---- carry: the partial sum (lower i face minus the j / k faces of this plane's
+    //      exchange), gradient record and centroid of the own cell I
+    if (act && inner) {
+      const int jup = tile_tid(tj + 1, tk), kup = kface_slot(tj, tk + 1);
 #pragma unroll
-    for (int v = 0; v < NFX; ++v) Fih[v] = Fin[v];
+      for (int v = 0; v < NFX; ++v)
+        Cp[v] = (double)Fin[v] - (((double)sFj[v][jup] - (double)sFj[v][tid]) +
+                                  ((double)sFk[v][kup] - (double)sFk[v][tid]));
+    }
     if (own_g) {
 #pragma unroll
       for (int v = 0; v < NGW; ++v) Gp[v] = sG[v][sg_own];
