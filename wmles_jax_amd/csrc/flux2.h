@@ -109,6 +109,49 @@ __device__ __forceinline__ float hybrid_weight(const NumDev& nd, float thL, floa
 // carries no runtime dispatch); SC_ANY reads everything from NumDev.
 // (the configurations SC_* and scheme_config are in common.h)
 
+// Kennedy-Gruber central flux of order 2 or 4 in FP32 (reference schemes.py:15-23,57-70),
+// the same quantities as central_flux<false, float> with fewer operations: each cell's
+// e + |u|^2 / 2 is formed once for the face's stencil, and the three two-point fluxes are
+// kept as pair sums (sr = rho_a + rho_b, su, sp, sE; w = su . S, M = sr w), whose scale
+// factors (1/4, 1/8, 1/2) fold into the order-4 combination:
+//   F = (8 A_LR - (A_LR2 + A_L2R)) / 48 per momentum / energy component with
+//   A_x = M su_x + 4 sp S_x, A_4 = M sE + 2 sp w; mass (8 M_LR - ...) / 24; u.S (8 w - ...) / 12.
+template <int ORDER>
+__device__ __forceinline__ void kep_pairs_f32(const CellWT<float>& L2, const CellWT<float>& L,
+                                              const CellWT<float>& Rc, const CellWT<float>& R2, float Sx,
+                                              float Sy, float Sz, float* F, float* un4) {
+  auto ek = [](const CellWT<float>& c) { return c.e + 0.5f * ((c.u * c.u + c.v * c.v) + c.w * c.w); };
+  const float EL = ek(L), ER = ek(Rc);
+  auto pair = [&](const CellWT<float>& a, const CellWT<float>& b, float Ea, float Eb, float* A, float* w) {
+    const float sr = a.r + b.r, su = a.u + b.u, sv = a.v + b.v, sw = a.w + b.w;
+    const float sp = a.p + b.p, sE = Ea + Eb;
+    const float ww = (su * Sx + sv * Sy) + sw * Sz;
+    const float M = sr * ww, sp4 = 4.f * sp;
+    A[0] = M;
+    A[1] = M * su + sp4 * Sx;
+    A[2] = M * sv + sp4 * Sy;
+    A[3] = M * sw + sp4 * Sz;
+    A[4] = M * sE + (2.f * sp) * ww;
+    *w = ww;
+  };
+  float A[5], w;
+  pair(L, Rc, EL, ER, A, &w);
+  if (ORDER == 2) {
+    F[0] = 0.25f * A[0];
+#pragma unroll
+    for (int c = 1; c < 5; ++c) F[c] = 0.125f * A[c];
+    *un4 = 0.5f * w;
+    return;
+  }
+  float Aa[5], wa, Ab[5], wb;
+  pair(L, R2, EL, ek(R2), Aa, &wa);
+  pair(L2, Rc, ek(L2), ER, Ab, &wb);
+  F[0] = (8.f * A[0] - (Aa[0] + Ab[0])) * (1.f / 24.f);
+#pragma unroll
+  for (int c = 1; c < 5; ++c) F[c] = (8.f * A[c] - (Aa[c] + Ab[c])) * (1.f / 48.f);
+  *un4 = (8.f * w - (wa + wb)) * (1.f / 12.f);
+}
+
 template <int SC>
 __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
                                               const CellWT<float>& L, const CellWT<float>& Rc,
@@ -128,7 +171,8 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
     const bool keep = SC == SC_ANY && (sch == WMX_SCHEME_KEEP ||
                                        (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP));
     if (keep) central_flux<true, float>(order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
-    else central_flux<false, float>(order, L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
+    else if (order == 2) kep_pairs_f32<2>(L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
+    else kep_pairs_f32<4>(L2, L, Rc, R2, sx, sy, sz, Fc, &un4);
   }
   const float wc = 1.f - s;
 #pragma unroll
