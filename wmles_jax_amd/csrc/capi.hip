@@ -412,6 +412,8 @@ NumDev make_numdev(const wmx_numerics_desc* nm) {
   std::memset(&nd, 0, sizeof(nd));
   nd.gas = GasC{nm->gamma, nm->R, nm->Pr, nm->Pr_t, nm->mu_ref, nm->T_ref, nm->S_suth,
                 nm->gamma * nm->R / (nm->gamma - 1.0)};
+  nd.rc = RecC{1.0 / nm->R, 1.0 / (nm->gamma - 1.0), (float)(1.0 / nm->T_ref),
+               (float)(nm->mu_ref * (nm->T_ref + nm->S_suth)), (float)nm->S_suth};
   nd.wm = WallModelC{nm->kappa, nm->B, nm->wm_ekb, nm->wm_pr13, nm->wm_pr3, nm->wm_beta, nm->n_iter};
   nd.scheme = nm->scheme;
   nd.central = nm->central;

@@ -69,8 +69,15 @@ struct WallNetDev {
   unsigned int* nfb;        // optional fallback counter
 };
 
+// host-evaluated constants of the FP32-bulk record store (rec32_store)
+struct RecC {
+  double iR, igm1;          // 1 / R, 1 / (gamma - 1)
+  float iTref, muc, S;      // 1 / T_ref, mu_ref (T_ref + S_suth), S_suth
+};
+
 struct NumDev {
   GasC gas;
+  RecC rc;
   WallModelC wm;
   int scheme, central, order, limiter;
   double th0, th1;
@@ -306,6 +313,31 @@ __device__ __forceinline__ float2 ld2(const float* rec) {
 __device__ __forceinline__ void st4(float* rec, float4 v) {
   wmx_f4 w = {v.x, v.y, v.z, v.w};
   *(__attribute__((address_space(1))) wmx_f4*)__builtin_assume_aligned(rec, 16) = w;
+}
+
+// FP32-bulk primitive record of a cell from its conserved state q, with the block gauge
+// (p' = p - p_ref, T' = T - T_gauge, e' = e_int - e_ref): cons_to_prim (reference
+// gas.py:28-33) in FP64 with one reciprocal of rho, and Sutherland's mu (gas.py:24-26)
+// from the float temperature (the record holds it as a float), stored as the two 16-byte
+// groups (u, v, w, T') and (rho, p', mu, e').  The arithmetic is not contracted (callers are
+// built with contraction off or, where fast, honour the pragma), so every caller stores the
+// same bits.
+__device__ __forceinline__ void rec32_store(float* Pb, size_t cells, size_t c, const NumDev& nd, const double* q,
+                                            double p_ref, double T_gauge, double e_ref) {
+#pragma clang fp contract(off)
+  const RecC& rc = nd.rc;
+  const double r = q[0];
+  const double ir = 1.0 / r;
+  const double u = q[1] * ir, v = q[2] * ir, w = q[3] * ir;
+  const double k = (u * u + v * v) + w * w;
+  const double p = (nd.gas.gamma - 1.0) * (q[4] - (0.5 * r) * k);
+  const double pr = p * ir;                      // p / rho = R T = (gamma - 1) e_int
+  const double T = pr * rc.iR;
+  const float Tf = (float)T;
+  const float t = Tf * rc.iTref;
+  const float mu = (rc.muc * (t * __builtin_amdgcn_sqrtf(t))) * __builtin_amdgcn_rcpf(Tf + rc.S);
+  st4(Pb + 4 * c, make_float4((float)u, (float)v, (float)w, (float)(T - T_gauge)));
+  st4(Pb + 4 * (cells + c), make_float4((float)r, (float)(p - p_ref), mu, (float)(pr * rc.igm1 - e_ref)));
 }
 
 // SmoothRampSponge weight at a cell centre x (smooth_ramp.py:212-221); pr = (x_start_l,

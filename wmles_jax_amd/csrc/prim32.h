@@ -253,13 +253,12 @@ __device__ __forceinline__ void prim32_block(const BlockDev* __restrict__ blks,
     const GasC& gas = nd.gas;
     double qv[5];
     prim_state(blks, b, c, q, gas, zfuse, qv);
-    const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
-    st4(B.Pf + 4 * (size_t)c, make_float4((float)w.u, (float)w.v, (float)w.w, (float)(w.T - B.T_gauge)));
-    st4(B.Pf + 4 * ((size_t)cells + c),
-        make_float4((float)w.r, (float)(w.p - B.p_ref), (float)mu_suth(gas, w.T),
-                    (float)(w.p / ((gas.gamma - 1.0) * w.r) - B.e_ref)));
+    rec32_store(B.Pf, (size_t)cells, (size_t)c, nd, qv, B.p_ref, B.T_gauge, B.e_ref);
     int I, J, K;
-    if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
+    if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) {
+      const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
+      dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
+    }
   }
   if (DT) {
     const double r = block_reduce<true>(dtc);
