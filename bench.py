@@ -425,12 +425,13 @@ def bench_wm(args, wmx):
                    "n_match": n_match},
                   _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload,
                             B_ALG_STEP["fp32"]))
-    # FP32-bulk plans evaluate the Newton faces inside the gradient launch (no launch of
+    # FP32-bulk plans evaluate the Newton faces in another launch's head (no launch of
     # their own); the learned model keeps its wall launch.  Both step times are reported.
     per = lambda pr: round(pr["wall"][0] / pr["wall"][1], 5) if "wall" in pr else None
     rec["wall_model"] = {
         "newton_ms_per_launch": per(prof_n),
-        "newton_in_gradient_launch": "wall" not in prof_n,
+        # the head of the primitive launch (fused form) or of the gradient launch (unfused)
+        "newton_in_shared_launch": "wall" not in prof_n,
         "learned_ms_per_launch": per(prof_l),
         "step_speedup_learned_vs_newton": round(el_n / el_l, 3),
         "fallback_fraction": round(fb / max(1, n_faces), 6), "n_match": n_match,
