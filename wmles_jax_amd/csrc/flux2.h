@@ -190,7 +190,7 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
   const float id = fdiv<float>(1.f, dist);
   const float phL[4] = {L.u, L.v, L.w, TL};
   const float phR[4] = {Rc.u, Rc.v, Rc.w, TR};
-  viscous_face_e<float>(gL, gR, phL, phR, d0 * id, d1 * id, d2 * id, dist, sx, sy, sz, Ff);
+  viscous_face_e<float>(gL, gR, phL, phR, d0 * id, d1 * id, d2 * id, id, sx, sy, sz, Ff);
 }
 
 template <class R>
@@ -233,11 +233,11 @@ __device__ __forceinline__ void flux_core(const NumDev& nd, const CellWT<R>& L2,
     for (int v = 0; v < 5; ++v) F[v] = Fc[v];
   }
   const double dx0 = xR[0] - xL[0], dx1 = xR[1] - xL[1], dx2 = xR[2] - xL[2];
-  const double dist = sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
+  const double idist = 1.0 / sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
   const R phL[4] = {L.u, L.v, L.w, TL};
   const R phR[4] = {Rc.u, Rc.v, Rc.w, TR};
   R Fv[5] = {R(0), R(0), R(0), R(0), R(0)};
-  viscous_face_e<R>(gL, gR, phL, phR, R(dx0 / dist), R(dx1 / dist), R(dx2 / dist), R(dist), sx, sy,
+  viscous_face_e<R>(gL, gR, phL, phR, R(dx0 * idist), R(dx1 * idist), R(dx2 * idist), R(idist), sx, sy,
                     sz, Fv);
 #pragma unroll
   for (int v = 0; v < 5; ++v) F[v] = F[v] + (double)Fv[v];

@@ -291,10 +291,12 @@ struct CellGT {
 using CellG = CellGT<double>;
 
 // reference viscous.py:28-51 for one face; phi = [u, v, w, T]; adds into F.
-// e (unit cell-to-cell vector) and dist come from FP64 centroids.
+// e (unit cell-to-cell vector) and idist = 1 / dist come from FP64 centroids (the four
+// differences phi_R - phi_L are scaled by one reciprocal: within an ulp of the reference's
+// division).
 template <class R>
 WMX_HD void viscous_face_e(const CellGT<R>& a, const CellGT<R>& b, const R* phiL, const R* phiR,
-                           R e0, R e1, R e2, R dist, R Sx, R Sy, R Sz, R* F) {
+                           R e0, R e1, R e2, R idist, R Sx, R Sy, R Sz, R* F) {
   const R h = R(0.5);
   R gf[12];
 #pragma unroll
@@ -302,7 +304,7 @@ WMX_HD void viscous_face_e(const CellGT<R>& a, const CellGT<R>& b, const R* phiL
     R g0 = h * (a.g[3 * v + 0] + b.g[3 * v + 0]);
     R g1 = h * (a.g[3 * v + 1] + b.g[3 * v + 1]);
     R g2 = h * (a.g[3 * v + 2] + b.g[3 * v + 2]);
-    R corr = fdiv<R>(phiR[v] - phiL[v], dist) - ((g0 * e0 + g1 * e1) + g2 * e2);
+    R corr = (phiR[v] - phiL[v]) * idist - ((g0 * e0 + g1 * e1) + g2 * e2);
     gf[3 * v + 0] = g0 + corr * e0;
     gf[3 * v + 1] = g1 + corr * e1;
     gf[3 * v + 2] = g2 + corr * e2;
@@ -329,8 +331,8 @@ WMX_HD void viscous_face(const CellG& a, const CellG& b, const double* phiL, con
                          const double* xcL, const double* xcR, double Sx, double Sy, double Sz,
                          double* F) {
   double dx0 = xcR[0] - xcL[0], dx1 = xcR[1] - xcL[1], dx2 = xcR[2] - xcL[2];
-  double dist = sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
-  viscous_face_e<double>(a, b, phiL, phiR, dx0 / dist, dx1 / dist, dx2 / dist, dist, Sx, Sy, Sz, F);
+  double idist = 1.0 / sqrt((dx0 * dx0 + dx1 * dx1) + dx2 * dx2);
+  viscous_face_e<double>(a, b, phiL, phiR, dx0 * idist, dx1 * idist, dx2 * idist, idist, Sx, Sy, Sz, F);
 }
 
 // Plain-form KEP/KEEP wrappers used by the FP64 kernels (no gauge).

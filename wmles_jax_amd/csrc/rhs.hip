@@ -65,9 +65,11 @@ __global__ __launch_bounds__(256) void k_grad_cell(const BlockDev* __restrict__ 
   }
   const double vol = B.vol[c];
   const R volr = R(vol);
+  // one reciprocal of the volume (within an ulp of the reference's twelve divisions)
+  const R ivol = fdiv<R>(R(1.0), volr);
   R g[12];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) g[i] = fdiv<R>(acc[i], volr);
+  for (int i = 0; i < 12; ++i) g[i] = acc[i] * ivol;
   const R rho = P[c];
   const R mu = P[P_MU * cells + c];
   R mut = R(0.0);
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(256) void k_grad_cell(const BlockDev* __restrict__ 
     // filter width vol^(1/3) (solver.py:82); FP32-bulk: single-precision cube root
     R delta;
     if constexpr (sizeof(R) == 4) delta = cbrtf((float)vol);
-    else delta = R(pow(vol, 1.0 / 3.0));
+    else delta = R(cbrt(vol));                   // vol ** (1/3), within an ulp
     const R cs = R(nd.sgs_c);
     R nut;
     if (nd.sgs == WMX_SGS_SMAGORINSKY) nut = nut_smagorinsky<R>(g, delta, cs);
