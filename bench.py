@@ -110,6 +110,14 @@ def profile_record(workload, fused=True):
     return {}
 
 
+def _own_stream():
+    """A new stream that first waits for the current one (the work it starts from)."""
+    import torch
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    return st
+
+
 def run_form(wmx, form, size, steps, warmup, breakdown, graph=False, fused=None):
     """Time `steps` steps of one arithmetic form; returns timings, final state, wall stats.
     fused: FP32-bulk kernel form (True: gradient records in the flux march; None: the
@@ -124,16 +132,19 @@ def run_form(wmx, form, size, steps, warmup, breakdown, graph=False, fused=None)
         s.set_fused_gradients(fused)
     pr = spec["props"]
     Q = s.init_state(lambda xc: cases.channel_prim(xc, pr, 0))
-    for _ in range(warmup):
-        Q = s.step(Q, graph=graph)[0]
-    torch.cuda.synchronize()
-    s.profile(s.KERNEL_CLASSES if breakdown else ("flux",))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        Q, dt = s.step(Q, graph=graph)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    # captured-graph steps replay on a stream of the caller's own (the legacy default stream
+    # cannot be captured), so the replay needs no side-stream handshake
+    with torch.cuda.stream(_own_stream() if graph else torch.cuda.current_stream()):
+        for _ in range(warmup):
+            Q = s.step(Q, graph=graph)[0]
+        torch.cuda.synchronize()
+        s.profile(s.KERNEL_CLASSES if breakdown else ("flux",))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            Q, dt = s.step(Q, graph=graph)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
     prof = s.profile_read()
     s.profile(())
     if graph:
@@ -303,7 +314,8 @@ def bench_ensemble(args, wmx):
     def ens_step(Qs):
         return ens.step(Qs, graph=args.graph)[0]
 
-    el_e, Qe = _timed_steps(ens_step, Qs0, args.steps, args.warmup)
+    with torch.cuda.stream(_own_stream() if args.graph else torch.cuda.current_stream()):
+        el_e, Qe = _timed_steps(ens_step, Qs0, args.steps, args.warmup)
     # flux launch time from events on eager (non-replayed) steps of the same plan
     ens.profile(("flux",))
     Qp = Qs0
@@ -324,7 +336,10 @@ def bench_ensemble(args, wmx):
     same = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qseq) for a, b in zip(Qa, Qb))
     # the other launch structures of the same plan: captured-graph replay, and one
     # persistent (cooperative) launch per RK stage
-    el_g, Qg = _timed_steps(lambda Qs: ens.step(Qs, graph=True)[0], Qs0, args.steps, args.warmup)
+    # (replayed on a stream of the caller's own, so no side-stream handshake: the caller's
+    # stream must not be the legacy default stream to be captured)
+    with torch.cuda.stream(_own_stream()):
+        el_g, Qg = _timed_steps(lambda Qs: ens.step(Qs, graph=True)[0], Qs0, args.steps, args.warmup)
     pers = wmx.EnsembleSolver(solvers)
     pers.set_tiling(*best)
     pers.set_persistent(True)

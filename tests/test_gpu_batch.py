@@ -23,16 +23,21 @@ def _channel(nblocks, nx, ny, nz, Re_b, seed, splits=None, precision="fp64"):
     return s, (lambda xc, pr=pr, seed=seed: cases.channel_prim(xc, pr, seed))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_graph_step_equals_max_dt_plus_advance(precision):
+@pytest.mark.parametrize("precision,own_stream", [("fp64", False), ("fp32", False), ("fp32", True)])
+def test_graph_step_equals_max_dt_plus_advance(precision, own_stream):
+    # own_stream: the caller runs on a stream of its own, which the step captures and
+    # replays on directly (on the legacy default stream it uses a side stream)
     s, fn = _channel(2, 32, 16, 16, 2e4, 0, precision=precision)
     Q0 = s.init_state(fn)
     A, B = Q0, Q0
-    for _ in range(4):                       # graph captured on the 1st/2nd call, replayed after
-        A, dta = s.step(A, graph=True)
-        dtb = s.max_dt(B)
-        B = s.advance(B, dtb)
-    torch.cuda.synchronize()
+    st = torch.cuda.Stream() if own_stream else torch.cuda.current_stream()
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        for _ in range(4):                   # graph captured on the 1st/2nd call, replayed after
+            A, dta = s.step(A, graph=True)
+            dtb = s.max_dt(B)
+            B = s.advance(B, dtb)
+        torch.cuda.synchronize()
     assert float(dta) == float(dtb)
     for a, b in zip(A, B):
         assert torch.equal(a, b)
@@ -49,8 +54,15 @@ def test_ensemble_equals_individual_cases():
     ens = wmx.EnsembleSolver(solvers)
     Qs = ens.init_state(fns)
     ref = [Q for Q in Qs]
-    for _ in range(3):
-        Qs, dts = ens.step(Qs, graph=True)
+    side = torch.cuda.Stream()
+    for it in range(3):
+        if it == 2:                          # the last step replayed on a stream of the caller's own
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                Qs, dts = ens.step(Qs, graph=True)
+            torch.cuda.current_stream().wait_stream(side)
+        else:
+            Qs, dts = ens.step(Qs, graph=True)
         new = []
         for c, (s, Q) in enumerate(zip(solvers, ref)):
             dt = s.max_dt(Q)

@@ -168,6 +168,11 @@ class EnsembleSolver:
                                          C.c_void_p(dt.data_ptr()), 0.0, 0, _lib.stream_ptr()))
             return self._split(out), dt[:self.n_cases]
         cur = torch.cuda.current_stream(self._dev)
+        if cur.cuda_stream != 0:      # the caller's capturable stream: no handshake
+            out = [torch.empty_like(q) for q in flat]
+            _lib.check(_lib.lib.wmx_step(self._plan, _lib.ptr_array(flat), _lib.ptr_array(out),
+                                         C.c_void_p(self._dt_dev.data_ptr()), 0.0, 1, _lib.stream_ptr(cur)))
+            return self._split(out), self._dt_dev[:self.n_cases].clone()
         self._stream.wait_stream(cur)
         with torch.cuda.stream(self._stream):
             out = [torch.empty_like(q) for q in flat]

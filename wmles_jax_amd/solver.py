@@ -410,12 +410,23 @@ class Solver:
             for c in self._py_constraints:
                 out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
             return out, dt[0]
-        # capture needs a stream other than the legacy default one: a side stream that
-        # keeps the dt buffer the graph was captured with
-        if not hasattr(self, "_stream"):
-            self._stream = torch.cuda.Stream(device=self._dev)
+        # capture needs a stream other than the legacy default one: the caller's own when it
+        # is one (no handshake), else a side stream; the dt buffer the graph was captured
+        # with is kept
+        if not hasattr(self, "_dt_dev"):
             self._dt_dev = torch.empty(16, dtype=torch.float64, device=self._dev)
         cur = torch.cuda.current_stream(self._dev)
+        if cur.cuda_stream != 0:
+            out = tuple(torch.empty_like(q) for q in Q)
+            _lib.check(_lib.lib.wmx_step(self._plan, _lib.ptr_array(Q), _lib.ptr_array(out),
+                                         C.c_void_p(self._dt_dev.data_ptr()), float(time), 1,
+                                         _lib.stream_ptr(cur)))
+            dt = self._dt_dev[0].clone()
+            for c in self._py_constraints:
+                out = tuple(self._as_state(c(out, self.geoms, self.blocks)))
+            return out, dt
+        if not hasattr(self, "_stream"):
+            self._stream = torch.cuda.Stream(device=self._dev)
         self._stream.wait_stream(cur)
         with torch.cuda.stream(self._stream):
             out = tuple(torch.empty_like(q) for q in Q)
