@@ -309,7 +309,10 @@ def bench_ensemble(args, wmx):
     nblk = sum(len(s.blocks) for s in solvers)
     ens = wmx.EnsembleSolver(solvers)
     Qs0 = ens.init_state(fns)
-    best, timings = autotune.tune(ens, Qs0, steps=3, use_cache=False)
+    if args.ens_forms == "eager":                # (no autotuner launches: the plan's own tiling)
+        best, timings = ens.tiling()[:2], {}
+    else:
+        best, timings = autotune.tune(ens, Qs0, steps=3, use_cache=False)
 
     def ens_step(Qs):
         return ens.step(Qs, graph=args.graph)[0]
@@ -332,6 +335,18 @@ def bench_ensemble(args, wmx):
     def seq_step(Qs):
         return [s.step(Q, graph=args.graph)[0] for s, Q in zip(solvers, Qs)]
 
+    workload = f"ensemble of 16 uneven channel cases ({nblk} blocks) KEP2 WALE WM fp32"
+    flux_ms, flux_n = prof["flux"]
+    rec = _record(args, cells * args.steps / el_e / 1e6, el_e * 1e3 / args.steps, "f32-bulk+f64", workload,
+                  {"cells": cells, "cases": 16, "blocks": nblk, "form": "fp32",
+                   "tiling": {"ci_len": best[0], "xcd_order": best[1]}},
+                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload,
+                            B_ALG_STEP["fp32"]))
+    rec["state_finite"] = all(bool(torch.isfinite(q).all()) for Q in Qe for q in Q)
+    rec["autotune_flux_ms"] = {k: round(v, 4) for k, v in timings.items()}
+    if args.ens_forms == "eager":
+        # the merged plan's eager steps only (a kernel trace of the ensemble launches alone)
+        return rec
     el_s, Qseq = _timed_steps(seq_step, Qs0, args.steps, args.warmup)
     same = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qseq) for a, b in zip(Qa, Qb))
     # the other launch structures of the same plan: captured-graph replay, and one
@@ -346,13 +361,6 @@ def bench_ensemble(args, wmx):
     el_p, Qpe = _timed_steps(lambda Qs: pers.step(Qs)[0], Qs0, args.steps, args.warmup)
     same_g = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qg) for a, b in zip(Qa, Qb))
     same_p = all(torch.equal(a, b) for Qa, Qb in zip(Qe, Qpe) for a, b in zip(Qa, Qb))
-    workload = f"ensemble of 16 uneven channel cases ({nblk} blocks) KEP2 WALE WM fp32"
-    flux_ms, flux_n = prof["flux"]
-    rec = _record(args, cells * args.steps / el_e / 1e6, el_e * 1e3 / args.steps, "f32-bulk+f64", workload,
-                  {"cells": cells, "cases": 16, "blocks": nblk, "form": "fp32",
-                   "tiling": {"ci_len": best[0], "xcd_order": best[1]}},
-                  _roofline(flux_ms / flux_n / 1e3, B_ALG_STEP["fp32"] / 3.0 * cells, workload,
-                            B_ALG_STEP["fp32"]))
     rec["sequential_value"] = round(cells * args.steps / el_s / 1e6, 2)
     rec["ensemble_speedup_vs_sequential"] = round(el_s / el_e, 3)
     rec["ensemble_equals_sequential"] = same
@@ -361,8 +369,6 @@ def bench_ensemble(args, wmx):
                                        "persistent_stage": round(el_p * 1e3 / args.steps, 4)}
     rec["graph_equals_eager"] = same_g
     rec["persistent_equals_eager"] = same_p
-    rec["state_finite"] = all(bool(torch.isfinite(q).all()) for Q in Qe for q in Q)
-    rec["autotune_flux_ms"] = {k: round(v, 4) for k, v in timings.items()}
     return rec
 
 
@@ -489,6 +495,9 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="time every kernel class (extra events)")
     ap.add_argument("--fused", default="auto", choices=["auto", "on", "off"],
                     help="FP32-bulk: gradient records in the flux march (on) or a separate kernel (off)")
+    ap.add_argument("--ens-forms", default="all", choices=["all", "eager"],
+                    help="ensemble16: also time the sequential cases, graph replay and persistent "
+                         "stages (all), or the merged plan's eager steps only (eager)")
     ap.add_argument("--graph", action="store_true",
                     help="replay captured hipGraphs for Solver.step (eager launches otherwise)")
     args = ap.parse_args()

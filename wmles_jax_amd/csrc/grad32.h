@@ -7,10 +7,16 @@
 
 namespace wmx {
 
+// block of workgroup wg: the largest b < nblk with prefix[b] <= wg (prefix non-decreasing,
+// prefix[0] = 0), by bisection (an ensemble plan has up to 64 blocks)
 __device__ __forceinline__ int find_block(const int* __restrict__ prefix, int nblk, int wg) {
-  int b = 0;
-  while (b + 1 < nblk && prefix[b + 1] <= wg) ++b;
-  return b;
+  int lo = 0, hi = nblk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 __device__ __forceinline__ double nan_min(double a, double b) {

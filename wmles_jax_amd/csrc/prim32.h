@@ -8,10 +8,16 @@
 
 namespace wmx {
 
+// block of workgroup wg: the largest b < nblk with prefix[b] <= wg (prefix non-decreasing,
+// prefix[0] = 0), by bisection (an ensemble plan has up to 64 blocks)
 __device__ __forceinline__ int find_block_r(const int* __restrict__ prefix, int nblk, int wg) {
-  int b = 0;
-  while (b + 1 < nblk && prefix[b + 1] <= wg) ++b;
-  return b;
+  int lo = 0, hi = nblk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 // NaN-propagating min (numpy.min semantics)
@@ -168,8 +174,7 @@ __device__ __forceinline__ void sponge_row_fast(const BlockDev& B, long long row
 __device__ __forceinline__ void sponge_head(const BlockDev* __restrict__ blks, int nblk, const Ptrs& q,
                                             const SpongeHead& sh, int h) {
   if (h >= sh.nwg) return;
-  int b = 0;
-  while (b + 1 < nblk && sh.pre[b + 1] <= h) ++b;
+  const int b = find_block_r(sh.pre, nblk, h);
   const BlockDev& B = blks[b];
   const long long rows = (long long)B.n[0] * B.n[1];
   const long long row = (long long)(h - sh.pre[b]) * 32 + (threadIdx.x >> 3);
