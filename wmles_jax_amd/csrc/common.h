@@ -43,7 +43,7 @@ struct BlockDev {
   float* Sf[3];             // FP32-bulk form: float copies of S[d] (gradient kernel)
   float* volf;              // FP32-bulk form: float copy of vol
   double* Sb;               // CFL metric planes (NSB_PLANES: the cell means of the face vectors
-                            // (d, x) and their norms), or nullptr: taken from S
+                            // (d, x)), or nullptr: taken from S
   double* wall[6];          // WALL_PLANES planes of (n_t1, n_t2) per WallModel face
   double* sponge_mean;      // (5, n0, n1)
   int* sponge_row;          // (n0, n1): 1 if some cell of the row has a sponge weight > 0
@@ -121,7 +121,7 @@ hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng
                          Ptrs q, GasC gas, double time, const double* d_dt, double coef,
                          hipStream_t s);
 hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
-constexpr int NSB_PLANES = 12;
+constexpr int NSB_PLANES = 9;
 hipError_t launch_cell_sb(const BlockDev& hb, hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);

@@ -58,11 +58,17 @@ __device__ __forceinline__ unsigned interior_cell(const BlockDev& B, unsigned l,
 
 // CFL metric of cell (I, J, K): the cell means of its face vectors
 // Sb[d] = 0.5 (S_lo + S_hi) in m[3 d + x] and their norms |Sb[d]| in m[9 + d]
-// (solver.py:145-146).  The per-cell geometry part of the CFL dt, which k_cell_sb stores at
-// workspace binding; cell_dt evaluates it itself when the plan has no such store (the same
-// arithmetic in the same unit, so the same bits).
+// (solver.py:145-146).  The per-cell geometry part of the CFL dt; k_cell_sb stores the
+// nine means at workspace binding (the norms are recomputed from them: 72 B per cell read
+// instead of 96), and cell_dt evaluates everything itself when the plan has no such store
+// (the same arithmetic in the same unit, so the same bits).
 constexpr int NSB = 12;
-static_assert(NSB == NSB_PLANES, "workspace planes of the CFL metric");
+static_assert(NSB_PLANES == 9, "workspace planes of the CFL metric: the nine cell-mean components");
+__device__ __forceinline__ void metric_norms(double* m) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    m[9 + d] = sqrt((m[3 * d] * m[3 * d] + m[3 * d + 1] * m[3 * d + 1]) + m[3 * d + 2] * m[3 * d + 2]);
+}
 __device__ __forceinline__ void cell_metric(const BlockDev& B, int I, int J, int K, double* m) {
   const int cc[3] = {I, J, K};
   double lo[3][3], hi[3][3];
@@ -81,11 +87,10 @@ __device__ __forceinline__ void cell_metric(const BlockDev& B, int I, int J, int
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < 3; ++d)
 #pragma unroll
     for (int x = 0; x < 3; ++x) m[3 * d + x] = 0.5 * (lo[d][x] + hi[d][x]);
-    m[9 + d] = sqrt((m[3 * d] * m[3 * d] + m[3 * d + 1] * m[3 * d + 1]) + m[3 * d + 2] * m[3 * d + 2]);
-  }
+  metric_norms(m);
 }
 
 // cell_dt's operands from the stored metric (planes B.Sb) and the volume of cell c
@@ -93,7 +98,8 @@ __device__ __forceinline__ void cell_dt_load(const BlockDev& B, unsigned c, doub
   const size_t cells = (size_t)B.cells;
   *vol = ldu(B.vol, c);
 #pragma unroll
-  for (int i = 0; i < NSB; ++i) m[i] = ldu(B.Sb + i * cells, c);
+  for (int i = 0; i < NSB_PLANES; ++i) m[i] = ldu(B.Sb + i * cells, c);
+  metric_norms(m);
 }
 
 // CFL time step of a cell from its primitives w, metric m and volume; solver.py:139-150, in

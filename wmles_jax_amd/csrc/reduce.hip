@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void k_cell_sb(BlockDev B) {
   double m[NSB];
   cell_metric(B, I, J, K, m);
 #pragma unroll
-  for (int i = 0; i < NSB; ++i) B.Sb[(size_t)i * cells + c] = m[i];
+  for (int i = 0; i < NSB_PLANES; ++i) B.Sb[(size_t)i * cells + c] = m[i];
 }
 
 hipError_t launch_cell_sb(const BlockDev& hb, hipStream_t s) {
