@@ -38,10 +38,37 @@ __device__ __forceinline__ CellG load_g(gconst<double> G, long long cells, long 
   return g;
 }
 
+// FP64 tile staging of one plane (k_flux1): gradient records on the tile + 1-cell ring
+// (17 x 18 cells, unpadded) and primitive records (r u v w p e T) on the tile + 2-cell ring
+// (19 x 20 cells)
+constexpr int NPW64 = 7;
+struct Stage64 {
+  const double (*sG)[NG2] = nullptr;   // [NGR][NG2]
+  int gL = -1, gR = -1;                // gradient slots of the left / right cell (-1: workspace)
+  const double (*sP)[NP2] = nullptr;   // [NPW64][NP2]
+  int pR = -1, pst = 0;                // primitive slot of the right cell (-1: workspace), stride
+};
+
+__device__ __forceinline__ CellW sp_w64(const double (*sP)[NP2], int s) {
+  CellW w;
+  w.r = sP[0][s]; w.u = sP[1][s]; w.v = sP[2][s]; w.w = sP[3][s]; w.p = sP[4][s]; w.e = sP[5][s];
+  return w;
+}
+
+__device__ __forceinline__ CellG sg_g64(const double (*sG)[NG2], int s) {
+  CellG g;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) g.g[i] = sG[i][s];
+  g.mu = sG[G_MU][s];
+  g.k = sG[G_K][s];
+  return g;
+}
+
 // Numerical flux (times area) through face f of direction d, f = coordinate of the
-// right cell (I, J, K); solver.py:88-93 with the wall override.
+// right cell (I, J, K); solver.py:88-93 with the wall override.  st: the tile's staged
+// records of the face's cells where it has them (else the workspace is read).
 __device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, int d, int I, int J,
-                                          int K, double* F) {
+                                          int K, double* F, const Stage64& st = Stage64()) {
   const int ng = B.ng;
   const int cc[3] = {I, J, K};
   const int f = cc[d];
@@ -67,21 +94,29 @@ __device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, i
   const long long fi = fidx(N, d, I, J, K), fc = fcount(N, d);
   const gconst<double> S = gmem(B.S[d]);
   const double Sx = S[fi], Sy = S[fc + fi], Sz = S[2 * fc + fi];
-  const CellW L = load_w(P, cells, cL), R = load_w(P, cells, cR);
+  const bool sp = st.pR >= 0;
+  const CellW L = sp ? sp_w64(st.sP, st.pR - st.pst) : load_w(P, cells, cL);
+  const CellW R = sp ? sp_w64(st.sP, st.pR) : load_w(P, cells, cR);
   const int sch = nd.scheme;
   if (sch == WMX_SCHEME_KEP || sch == WMX_SCHEME_KEEP) {
     CellW L2 = L, R2 = R;
-    if (nd.order == 4) { L2 = load_w(P, cells, cL - sd); R2 = load_w(P, cells, cR + sd); }
+    if (nd.order == 4) {
+      L2 = sp ? sp_w64(st.sP, st.pR - 2 * st.pst) : load_w(P, cells, cL - sd);
+      R2 = sp ? sp_w64(st.sP, st.pR + st.pst) : load_w(P, cells, cR + sd);
+    }
     if (sch == WMX_SCHEME_KEP) central_flux<false>(nd.order, L2, L, R, R2, Sx, Sy, Sz, F);
     else central_flux<true>(nd.order, L2, L, R, R2, Sx, Sy, Sz, F);
   } else {
-    const CellW L2 = load_w(P, cells, cL - sd), R2 = load_w(P, cells, cR + sd);
+    const CellW L2 = sp ? sp_w64(st.sP, st.pR - 2 * st.pst) : load_w(P, cells, cL - sd);
+    const CellW R2 = sp ? sp_w64(st.sP, st.pR + st.pst) : load_w(P, cells, cR + sd);
     if (sch == WMX_SCHEME_MUSCLHLLC) {
       muscl_hllc_flux(nd.gas.gamma, nd.limiter, L2, L, R, R2, Sx, Sy, Sz, F);
     } else {
       // Hybrid: (1 - s) F_c + s F_u; a part whose weight is exactly 0 is not
       // evaluated (bit-identical while the skipped flux is finite).
-      const double th = fmax(G[G_TH * cells + cL], G[G_TH * cells + cR]);
+      const double thL = st.gL >= 0 ? st.sG[G_TH][st.gL] : G[G_TH * cells + cL];
+      const double thR = st.gR >= 0 ? st.sG[G_TH][st.gR] : G[G_TH * cells + cR];
+      const double th = fmax(thL, thR);
       const double s = fmin(fmax((th - nd.th0) / (nd.th1 - nd.th0), 0.0), 1.0);
       double Fc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, Fu[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       if (s < 1.0) {
@@ -94,9 +129,10 @@ __device__ __forceinline__ void face_flux(const NumDev& nd, const BlockDev& B, i
     }
   }
   // viscous part (viscous.py:28-51)
-  const CellG gL = load_g(G, cells, cL), gR = load_g(G, cells, cR);
-  const double phL[4] = {L.u, L.v, L.w, P[P_T * cells + cL]};
-  const double phR[4] = {R.u, R.v, R.w, P[P_T * cells + cR]};
+  const CellG gL = st.gL >= 0 ? sg_g64(st.sG, st.gL) : load_g(G, cells, cL);
+  const CellG gR = st.gR >= 0 ? sg_g64(st.sG, st.gR) : load_g(G, cells, cR);
+  const double phL[4] = {L.u, L.v, L.w, sp ? st.sP[6][st.pR - st.pst] : P[P_T * cells + cL]};
+  const double phR[4] = {R.u, R.v, R.w, sp ? st.sP[6][st.pR] : P[P_T * cells + cR]};
   const gconst<double> XC = gmem(B.xc);
   const double xL[3] = {XC[cL], XC[cells + cL], XC[2 * cells + cL]};
   const double xR[3] = {XC[cR], XC[cells + cR], XC[2 * cells + cR]};
@@ -117,6 +153,8 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
                                               const double* __restrict__ d_dt, double dt_host) {
   __shared__ double sFj[5][(C2J + 1) * TK];
   __shared__ double sFk[5][C2J * (TK + 1)];
+  __shared__ double sG[NGR][NG2];                // gradient records of plane I, tile + 1-cell ring
+  __shared__ double sP[NPW64][NP2];              // primitives of plane I, tile + 2-cell ring
   const int4 t = tiles[blockIdx.x];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
@@ -134,8 +172,45 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
   const long long cells = B.cells;
   double Flo[5], Fhi[5];
   if (act) face_flux(nd, B, 0, ng + i0, J, K, Flo);
+  const gconst<double> Gw = gmem(B.G);
+  const gconst<double> Pw = gmem(B.P);
+  const int pl[NPW64] = {P_R, P_U, P_V, P_W, P_P, P_E, P_T};
   for (int i = i0; i < i1; ++i) {
     const int I = ng + i;
+    // plane I staged for the j / k faces and the lower cell of the i face: the gradient
+    // records on the tile + 1-cell ring (rows j0-1 .. j0+15, columns k0-1 .. k0+16, within
+    // the cells the gradient launch covers) and the primitives on the tile + 2-cell ring
+    // (within the array)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int r = tid + u * NT;
+      if (r < NG2) {
+        const int gj = r / G2K, gk = r - gj * G2K;
+        const int JJ = ng + j0 - 1 + gj, KK = ng + k0 - 1 + gk;
+        if (JJ <= ng + B.n[1] && KK <= ng + B.n[2]) {
+          const long long c = cidx(B.N, I, JJ, KK);
+          double v[NGR];
+#pragma unroll
+          for (int x = 0; x < NGR; ++x) v[x] = Gw[x * cells + c];
+#pragma unroll
+          for (int x = 0; x < NGR; ++x) sG[x][r] = v[x];
+        }
+      }
+      const int rp = tid + u * NT;
+      if (rp < NP2) {
+        const int pj = rp / H2K, pk = rp - pj * H2K;
+        const int JJ = ng + j0 - 2 + pj, KK = ng + k0 - 2 + pk;
+        if (JJ < B.N[1] && KK < B.N[2] && JJ >= 0 && KK >= 0) {
+          const long long c = cidx(B.N, I, JJ, KK);
+          double v[NPW64];
+#pragma unroll
+          for (int x = 0; x < NPW64; ++x) v[x] = Pw[pl[x] * cells + c];
+#pragma unroll
+          for (int x = 0; x < NPW64; ++x) sP[x][rp] = v[x];
+        }
+      }
+    }
+    __syncthreads();
 #pragma unroll 1
     for (int task = 0; task < 3; ++task) {
       int d, fI = I, rj = tj, rk = tk;
@@ -148,7 +223,20 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
       } else { d = 0; fI = I + 1; doit = act; }
       double F[5];
       if (doit) {
-        face_flux(nd, B, d, fI, ng + j0 + rj, ng + k0 + rk, F);
+        // staged slots of the face's cells (the i face's stencil reaches planes I - 1 .. I + 2:
+        // its primitives and its upper cell's gradient record come from the workspace)
+        Stage64 st;
+        st.sG = sG;
+        st.sP = sP;
+        if (d == 0) {
+          st.gL = (tj + 1) * G2K + (tk + 1);
+        } else {
+          st.gR = (rj + 1) * G2K + (rk + 1);
+          st.gL = d == 1 ? rj * G2K + (rk + 1) : (rj + 1) * G2K + rk;
+          st.pR = (rj + 2) * H2K + (rk + 2);
+          st.pst = d == 1 ? H2K : 1;
+        }
+        face_flux(nd, B, d, fI, ng + j0 + rj, ng + k0 + rk, F, st);
         if (task == 0) {
 #pragma unroll
           for (int v = 0; v < 5; ++v) sFj[v][rj * TK + rk] = F[v];
