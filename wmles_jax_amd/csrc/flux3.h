@@ -395,6 +395,13 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
     }
     if (inner) __syncthreads();                  // (uniform) j / k face exchange
     if (fin) {
+      // stages 2 and 3: the base-state loads go out first, so the sums below overlap their
+      // latency (before the barrier their registers would spill)
+      if (MODE >= 2) {
+        const double* qz = q0.p[t.x];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) vz[v] = ldu(qz + v * cells, c);
+      }
       ivol = 1.0 / ivol;
       // outward sums in FP64 of the FP32 face values; the energy gets the gauge terms
       // e_ref x (mass divergence) + p_ref x (sum of w)
@@ -428,9 +435,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
 #pragma unroll
         for (int v = 0; v < 5; ++v) stu(o + v * cells, c, vq[v] + dt * r[v]);
       } else {
-        const double* qz = q0.p[t.x];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) vz[v] = ldu(qz + v * cells, c);
         const double a0 = MODE == 2 ? 0.75 : 1.0 / 3.0;
         const double a1 = MODE == 2 ? 0.25 : 2.0 / 3.0;
 #pragma unroll
