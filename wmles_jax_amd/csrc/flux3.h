@@ -383,6 +383,15 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         }
       }
     }
+    // ---- prefetch for plane I+1 (own record of I+2, own centroid of I+1, the ring), issued
+    //      before the exchange barrier so that the epilogue and the barrier cover its
+    //      latency (this plane's staging consumed the registers it overwrites)
+    if (I < Iend) {
+      Pn = load_rec32(P, cells, (size_t)(I + 2) * sI + cb);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) Xn[x] = ldu(XC + x * (size_t)cells, (unsigned)(I + 1) * sI + cb);
+      ring_load(I + 2, I + 1);
+    }
     // ---- cell I-1: its state and volume loads go out before the exchange barrier
     const bool fin = act && I >= I0 + 1;
     const unsigned c = (unsigned)(I - 1) * sI + (unsigned)J * N[2] + K;
@@ -455,13 +464,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       for (int v = 0; v < NGW; ++v) Gp[v] = sG[v][sg_own];
 #pragma unroll
       for (int x = 0; x < 3; ++x) Xp[x] = sX[x][sg_own];
-    }
-    // ---- prefetch for plane I+1: own record of I+2, own centroid of I+1, the ring
-    if (I < Iend) {
-      Pn = load_rec32(P, cells, (size_t)(I + 2) * sI + cb);
-#pragma unroll
-      for (int x = 0; x < 3; ++x) Xn[x] = ldu(XC + x * (size_t)cells, (unsigned)(I + 1) * sI + cb);
-      ring_load(I + 2, I + 1);
     }
   }
 }
