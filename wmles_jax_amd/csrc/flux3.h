@@ -299,6 +299,17 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
     __syncthreads();
     // ---- face fluxes: 0 j face below row tj, 1 k face left of column tk (or the upper
     //      edge k face of row tk), 2 the own i face I-1/2
+    // ---- cell I-1: its state and volume loads go out before the face tasks, whose
+    //      arithmetic covers their latency
+    const bool fin = act && I >= I0 + 1;
+    const unsigned c = (unsigned)(I - 1) * sI + (unsigned)J * N[2] + K;
+    const double* qi = qin.p[t.x];
+    double vq[5], vz[5], ivol = 0.0;
+    if (fin) {
+      ivol = ldu(B.vol, c);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) vq[v] = ldu(qi + v * cells, c);
+    }
     float Fin[NFX] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
     for (int task = inner ? 0 : 2; task < (I >= I0 ? 3 : 0); ++task) {
@@ -391,16 +402,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
 #pragma unroll
       for (int x = 0; x < 3; ++x) Xn[x] = ldu(XC + x * (size_t)cells, (unsigned)(I + 1) * sI + cb);
       ring_load(I + 2, I + 1);
-    }
-    // ---- cell I-1: its state and volume loads go out before the exchange barrier
-    const bool fin = act && I >= I0 + 1;
-    const unsigned c = (unsigned)(I - 1) * sI + (unsigned)J * N[2] + K;
-    const double* qi = qin.p[t.x];
-    double vq[5], vz[5], ivol = 0.0;
-    if (fin) {
-      ivol = ldu(B.vol, c);
-#pragma unroll
-      for (int v = 0; v < 5; ++v) vq[v] = ldu(qi + v * cells, c);
     }
     if (inner) __syncthreads();                  // (uniform) j / k face exchange
     if (fin) {
