@@ -369,6 +369,7 @@ WMX_HD long long fcount(const int* N, int d) {
 }
 
 // ghost shell cell s -> (I, J, K): i-slabs (full j,k), j-slabs (interior i), k-slabs
+// (interior i, j); k fastest within each
 WMX_HD void shell_cell(long long s, const int* N, const int* n, int ng, int* I, int* J, int* K) {
   long long a = 2LL * ng * N[1] * N[2];
   if (s < a) {
@@ -391,12 +392,16 @@ WMX_HD void shell_cell(long long s, const int* N, const int* n, int ng, int* I, 
     return;
   }
   s -= b;
-  long long per = (long long)n[0] * n[1];
-  long long l = s / per;
-  long long r = s % per;
-  *I = ng + (int)(r / n[1]);
-  *J = ng + (int)(r % n[1]);
-  *K = l < ng ? (int)l : (int)(n[2] + l);
+  // k-slabs: side, then interior (I, J) row, then the ng ghost layers of the row (contiguous
+  // along k, so a wavefront touches ng-cell runs instead of one cell per row)
+  long long per = (long long)n[0] * n[1] * ng;
+  const int side = s < per ? 0 : 1;
+  long long t = s - side * per;
+  long long row = t / ng;
+  const int m = (int)(t - row * ng);
+  *I = ng + (int)(row / n[1]);
+  *J = ng + (int)(row % n[1]);
+  *K = side == 0 ? m : ng + n[2] + m;
 }
 
 // k ghost of shell cell (I, J, K) whose side the step's primitive launches evaluate (the
