@@ -152,21 +152,48 @@ __device__ __forceinline__ void kep_pairs_f32(const CellWT<float>& L2, const Cel
   *un4 = (8.f * w - (wa + wb)) * (1.f / 12.f);
 }
 
+// Hybrid weight s of a face of scheme configuration SC (0: central only, 1: upwind only).
 template <int SC>
-__device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
-                                              const CellWT<float>& L, const CellWT<float>& Rc,
-                                              const CellWT<float>& R2, float thL, float thR,
-                                              const CellGT<float>& gL, const CellGT<float>& gR,
-                                              float TL, float TR, float d0, float d1, float d2,
-                                              float sx, float sy, float sz, float pr, float er,
-                                              float* Ff, float* uw) {
+__device__ __forceinline__ float face_weight(const NumDev& nd, float thL, float thR) {
+  const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
+                  : (SC == SC_KEP2 || SC == SC_KEP4) ? WMX_SCHEME_KEP : nd.scheme;
+  float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
+  if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
+  return s;
+}
+
+// Upwind part of a face with weight s > 0: Ff += s Fu in gauge form (momentum - p_ref S,
+// energy - e_ref Fu[0]), Fu the MUSCL-HLLC flux with the full pressure.
+template <int SC>
+__device__ __forceinline__ void upwind_add(const NumDev& nd, float s, const CellWT<float>& L2,
+                                           const CellWT<float>& L, const CellWT<float>& Rc,
+                                           const CellWT<float>& R2, float sx, float sy, float sz, float pr,
+                                           float er, float* Ff) {
+  const int limiter = SC == SC_HYB_KEP4_ALBADA ? 0 : nd.limiter;
+  float Fu[5];
+  muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz, pr, Fu);
+  Ff[0] = Ff[0] + s * Fu[0];
+  Ff[1] = Ff[1] + s * (Fu[1] - pr * sx);
+  Ff[2] = Ff[2] + s * (Fu[2] - pr * sy);
+  Ff[3] = Ff[3] + s * (Fu[3] - pr * sz);
+  Ff[4] = Ff[4] + s * (Fu[4] - er * Fu[0]);
+}
+
+// FP32-bulk face flux: (1 - s) central + s upwind + viscous, returns s.  DEFER: the upwind
+// part is left to the caller (upwind_add after the viscous part).
+template <int SC, bool DEFER = false>
+__device__ __forceinline__ float flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
+                                               const CellWT<float>& L, const CellWT<float>& Rc,
+                                               const CellWT<float>& R2, float thL, float thR,
+                                               const CellGT<float>& gL, const CellGT<float>& gR,
+                                               float TL, float TR, float d0, float d1, float d2,
+                                               float sx, float sy, float sz, float pr, float er,
+                                               float* Ff, float* uw) {
   const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
                   : (SC == SC_KEP2 || SC == SC_KEP4) ? WMX_SCHEME_KEP : nd.scheme;
   const int order = SC == SC_KEP2 ? 2 : ((SC == SC_HYB_KEP4_ALBADA || SC == SC_KEP4) ? 4 : nd.order);
-  const int limiter = SC == SC_HYB_KEP4_ALBADA ? 0 : nd.limiter;
-  float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
-  if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
-  float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Fu[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
+  const float s = face_weight<SC>(nd, thL, thR);
+  float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
   if (s < 1.f) {
     const bool keep = SC == SC_ANY && (sch == WMX_SCHEME_KEEP ||
                                        (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP));
@@ -177,20 +204,14 @@ __device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<flo
   const float wc = 1.f - s;
 #pragma unroll
   for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v];
-  if (s > 0.f) {
-    muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz, pr, Fu);
-    Ff[0] = Ff[0] + s * Fu[0];
-    Ff[1] = Ff[1] + s * (Fu[1] - pr * sx);
-    Ff[2] = Ff[2] + s * (Fu[2] - pr * sy);
-    Ff[3] = Ff[3] + s * (Fu[3] - pr * sz);
-    Ff[4] = Ff[4] + s * (Fu[4] - er * Fu[0]);
-  }
+  if (!DEFER && s > 0.f) upwind_add<SC>(nd, s, L2, L, Rc, R2, sx, sy, sz, pr, er, Ff);
   *uw = wc * un4;
   const float dist = fsqrt<float>((d0 * d0 + d1 * d1) + d2 * d2);
   const float id = fdiv<float>(1.f, dist);
   const float phL[4] = {L.u, L.v, L.w, TL};
   const float phR[4] = {Rc.u, Rc.v, Rc.w, TR};
   viscous_face_e<float>(gL, gR, phL, phR, d0 * id, d1 * id, d2 * id, id, sx, sy, sz, Ff);
+  return s;
 }
 
 template <class R>

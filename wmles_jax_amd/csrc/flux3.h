@@ -38,6 +38,11 @@ __device__ __forceinline__ Rec32 load_rec32(const float* P, size_t cells, size_t
   return o;
 }
 
+// number of lanes below this one whose bit is set in the wavefront mask m
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 __device__ __forceinline__ CellWT<float> rec_w(const Rec32& a) {
   CellWT<float> w;
   w.r = a.r; w.u = a.u; w.v = a.v; w.w = a.w; w.p = a.p; w.e = a.e;
@@ -144,12 +149,14 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
   __shared__ float sS[3][3][NG2S];               // lower i / j / k face vectors of the gradient cells
   __shared__ float sFj[NFX][NT];                 // slot tile_tid(r, c)
   __shared__ float sFk[NFX][NT + C2J + 1];       // slot kface_slot(r, c)
+  __shared__ unsigned short sUq[NT / 64][2 * 64];  // per wavefront: j / k faces with upwind work
   const int4 t = tiles[tile];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
   const int i1 = min(i0 + ci_len, B.n[0]);
   const int tid = threadIdx.x;
   const int tj = (tid >> 6) + 4 * ((tid >> 4) & 3), tk = tid & 15;   // tile_tid(tj, tk) == tid
+  const int wv = tid >> 6, lane = tid & 63;
   const int ng = B.ng;
   const int* N = B.N;
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
@@ -311,9 +318,16 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       for (int v = 0; v < 5; ++v) vq[v] = ldu(qi + v * cells, c);
     }
     float Fin[NFX] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // Hybrid configurations: the tasks evaluate the central and viscous parts; the upwind
+    // (MUSCL-HLLC) parts of the faces with s > 0 are evaluated afterwards in packed passes
+    // of the wavefront (the j / k faces listed in sUq, the own i face flagged in up_i), so
+    // that a wavefront runs the upwind arithmetic once per plane instead of once per task
+    constexpr bool DEFER = SC == SC_HYB_KEP4_ALBADA || SC == SC_ANY;
+    int nq = 0;                                  // listed faces of the wavefront (uniform)
+    bool up_i = false;
 #pragma unroll 1
     for (int task = inner ? 0 : 2; task < (I >= I0 ? 3 : 0); ++task) {
-      bool doit = act;
+      bool doit = act, later = false;
       int d = 0, fI = I, fJ = J, fK = K;
       int rj = tj, rk = tk;
       CellWT<float> L2, L, Rc, R2;
@@ -379,8 +393,10 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         } else {
           // the face vector, staged by the gradient cell whose lower face it is
           const int ss = task == 2 ? sg_own : (rj + 1) * G2P + (rk + 1);
-          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, dx[0], dx[1], dx[2], sS[d][0][ss], sS[d][1][ss],
-                            sS[d][2][ss], float(B.p_ref), float(B.e_ref), F, &F[5]);
+          const float sw = flux_core_f32<SC, DEFER>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, dx[0], dx[1], dx[2],
+                                                    sS[d][0][ss], sS[d][1][ss], sS[d][2][ss], float(B.p_ref),
+                                                    float(B.e_ref), F, &F[5]);
+          later = DEFER && sw > 0.f;
         }
         if (task == 2) {
 #pragma unroll
@@ -392,6 +408,68 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
 #pragma unroll
           for (int v = 0; v < NFX; ++v) sFk[v][kface_slot(rj, rk)] = F[v];
         }
+      }
+      if (DEFER) {
+        if (task == 2) {
+          up_i = later;
+        } else {
+          const unsigned long long m = __ballot(later);
+          if (later) sUq[wv][nq + lanes_below(m)] = (unsigned short)((d << 10) | (rj << 5) | rk);
+          nq += __popcll(m);
+        }
+      }
+    }
+    if (DEFER) {
+      // packed upwind passes: in the first, the lanes whose i face has upwind work take it
+      // and the other lanes the first listed faces; then 64 listed faces per pass
+      const unsigned long long mi = __ballot(up_i);
+      const int ni = __popcll(mi), rank_free = lanes_below(~mi);
+      const float pr = float(B.p_ref), er = float(B.e_ref);
+      for (int qb = 0, pass = 0; (pass == 0 && ni > 0) || qb < nq; ++pass) {
+        const bool mine = pass == 0 && up_i;
+        const int q = qb + (pass == 0 ? rank_free : lane);
+        const bool lst = !mine && q < nq;
+        if (mine || lst) {
+          CellWT<float> L2, L, Rc, R2;
+          float thL, thR, sv[3];
+          int dd = 0, rj = 0, rk = 0;
+          if (mine) {
+            L2 = rec_w(Pw[0]); L = rec_w(Pw[1]); Rc = rec_w(Pw[2]); R2 = rec_w(Pw[3]);
+            thL = Gp[14]; thR = sG[14][sg_own];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) sv[x] = sS[0][x][sg_own];
+          } else {
+            const int dsc = sUq[wv][q];
+            dd = dsc >> 10; rj = (dsc >> 5) & 31; rk = dsc & 31;
+            const int pR = (rj + 2) * H2K + (rk + 2), gR_ = (rj + 1) * G2P + (rk + 1);
+            const int pst = dd == 1 ? H2K : 1, gst = dd == 1 ? G2P : 1;
+            auto cw = [&](int s) {
+              CellWT<float> w;
+              w.r = sPr[0][s]; w.u = sPh[s_0][0][s]; w.v = sPh[s_0][1][s]; w.w = sPh[s_0][2][s];
+              w.p = sPr[1][s]; w.e = sPr[3][s];
+              return w;
+            };
+            L2 = cw(pR - 2 * pst); L = cw(pR - pst); Rc = cw(pR); R2 = cw(pR + pst);
+            thL = sG[14][gR_ - gst]; thR = sG[14][gR_];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) sv[x] = sS[dd][x][gR_];
+          }
+          float dF[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+          upwind_add<SC>(nd, face_weight<SC>(nd, thL, thR), L2, L, Rc, R2, sv[0], sv[1], sv[2], pr, er, dF);
+          if (mine) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Fin[v] = Fin[v] + dF[v];
+          } else if (dd == 1) {
+            const int sl = tile_tid(rj, rk);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sFj[v][sl] = sFj[v][sl] + dF[v];
+          } else {
+            const int sl = kface_slot(rj, rk);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sFk[v][sl] = sFk[v][sl] + dF[v];
+          }
+        }
+        qb += pass == 0 ? 64 - ni : 64;
       }
     }
     // ---- prefetch for plane I+1 (own record of I+2, own centroid of I+1, the ring), issued
