@@ -49,34 +49,79 @@ __device__ __forceinline__ CellWT<float> rec_w(const Rec32& a) {
   return w;
 }
 
-// Gradient record of cell (I, J, K) (P-region slot ps, gradient-region slot gs) from the
-// staged planes: the per-cell arithmetic of k_grad32 (grad32.h), faces in the order i, j, k.
-__device__ __forceinline__ void grad_cell3(const BlockDev& B, const NumDev& nd, const float (*pm)[NP2],
-                                           const float (*p0)[NP2], const float (*pp)[NP2],
-                                           const float (*pr)[NP2], int ps, int I, int J, int K, float* G,
-                                           float* Slo) {
-  const int N1 = B.N[1], N2 = B.N[2];
-  const unsigned st0 = (unsigned)N1 * N2;
+// Face-vector slots of a tile in LDS: the lower i / j / k face vectors of the cells of the
+// gradient region's rows -1 .. 16 (18 rows of pitch G2P; slot gj * G2P + gk), so that the
+// upper j / k faces of the gradient cells are the lower faces of the slots one row / column
+// on.  They arrive by direct-to-LDS loads (global_load_lds_dword: lane l of a wavefront
+// writes slot 64 w' + l of the instruction's run), issued a plane ahead.  NSS covers the
+// runs of the last wavefront that loads (360 slots used).
+constexpr int NSS_USED = (G2J + 1) * G2P, NSS = 384;
+// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for (gfx9 encoding: vmcnt
+// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr unsigned WAITCNT_VM0 = 0x0F70;
+static_assert(NSS_USED <= NSS && NSS % 64 == 0 && NSS - NT <= 128, "face-vector slots");
+
+// Upper i face vector and FP32 volume of a gradient cell (the part of its geometry that is
+// not in the face-vector slots).
+struct GeoHi {
+  float S0hi[3], volr;
+};
+
+__device__ __forceinline__ void geo_hi_load(const BlockDev& B, int I, int J, int K, GeoHi& q) {
+  const unsigned st0 = (unsigned)B.N[1] * B.N[2];
+  const unsigned fc0 = (unsigned)fcount(B.N, 0);
+  const unsigned c = (unsigned)I * st0 + (unsigned)J * B.N[2] + (unsigned)K;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) q.S0hi[x] = ldu(B.Sf[0] + x * (size_t)fc0, c + st0);
+  q.volr = ldu((const float*)B.volf, c);
+}
+
+// Direct-to-LDS loads of plane I's face-vector slots (slot s -> cell (s / G2P, s % G2P) of
+// the region, indices clamped into the block: slots that no gradient cell reads may hold
+// any cell's values).  The data lands when the loads complete; the workgroup barrier that
+// follows them waits for that (vmcnt) before any wavefront reads the slots.
+__device__ __forceinline__ void face_dma(const BlockDev& B, int I, int j0, int k0, float (*sS)[3][NSS], int tid) {
+  const int ng = B.ng, N1 = B.N[1], N2 = B.N[2];
   const unsigned fc0 = (unsigned)fcount(B.N, 0), fc1 = (unsigned)fcount(B.N, 1), fc2 = (unsigned)fcount(B.N, 2);
-  const unsigned c = (unsigned)I * st0 + (unsigned)J * N2 + (unsigned)K;
-  const unsigned f0 = c;
-  const unsigned f1 = (unsigned)I * ((unsigned)(N1 + 1) * N2) + (unsigned)J * N2 + (unsigned)K;
-  const unsigned f2 = (unsigned)I * ((unsigned)N1 * (N2 + 1)) + (unsigned)J * (unsigned)(N2 + 1) + (unsigned)K;
-  const float* S0 = B.Sf[0];
-  const float* S1 = B.Sf[1];
-  const float* S2 = B.Sf[2];
-  // every load first (one memory round trip), then the arithmetic
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int s = tid + b * NT;
+    if (b == 1 && tid >= NSS - NT) break;                        // (uniform for whole runs)
+    const int gj = s / G2P, gk = s - gj * G2P;
+    const int J = min(max(ng + j0 - 1 + gj, 0), N1 - 1), K = min(max(ng + k0 - 1 + gk, 0), N2 - 1);
+    const unsigned f0 = (unsigned)I * ((unsigned)N1 * N2) + (unsigned)J * N2 + (unsigned)K;
+    const unsigned f1 = (unsigned)I * ((unsigned)(N1 + 1) * N2) + (unsigned)J * N2 + (unsigned)K;
+    const unsigned f2 = (unsigned)I * ((unsigned)N1 * (N2 + 1)) + (unsigned)J * (unsigned)(N2 + 1) + (unsigned)K;
+    const int run = s & ~63;                                     // the wavefront's 64-slot run
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      __builtin_amdgcn_global_load_lds((const void*)(B.Sf[0] + x * (size_t)fc0 + f0),
+                                       (__attribute__((address_space(3))) void*)&sS[0][x][run], 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(B.Sf[1] + x * (size_t)fc1 + f1),
+                                       (__attribute__((address_space(3))) void*)&sS[1][x][run], 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(B.Sf[2] + x * (size_t)fc2 + f2),
+                                       (__attribute__((address_space(3))) void*)&sS[2][x][run], 4, 0, 0);
+    }
+  }
+}
+
+// Gradient record of a cell (P-region slot ps, face-vector slot gsl) from the staged planes,
+// the face-vector slots and q: the per-cell arithmetic of k_grad32 (grad32.h), faces in the
+// order i, j, k.
+__device__ __forceinline__ void grad_cell3(const NumDev& nd, const GeoHi& q, const float (*sS)[3][NSS], int gsl,
+                                           const float (*pm)[NP2], const float (*p0)[NP2], const float (*pp)[NP2],
+                                           const float (*pr)[NP2], int ps, float* G) {
   float S0lo[3], S0hi[3], S1lo[3], S1hi[3], S2lo[3], S2hi[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    S0lo[x] = ldu(S0 + x * (size_t)fc0, f0);
-    S0hi[x] = ldu(S0 + x * (size_t)fc0, f0 + st0);
-    S1lo[x] = ldu(S1 + x * (size_t)fc1, f1);
-    S1hi[x] = ldu(S1 + x * (size_t)fc1, f1 + (unsigned)N2);
-    S2lo[x] = ldu(S2 + x * (size_t)fc2, f2);
-    S2hi[x] = ldu(S2 + x * (size_t)fc2, f2 + 1);
+    S0lo[x] = sS[0][x][gsl];
+    S0hi[x] = q.S0hi[x];
+    S1lo[x] = sS[1][x][gsl];
+    S1hi[x] = sS[1][x][gsl + G2P];
+    S2lo[x] = sS[2][x][gsl];
+    S2hi[x] = sS[2][x][gsl + 1];
   }
-  const float volr = ldu((const float*)B.volf, c);
+  const float volr = q.volr;
   const float h = 0.5f;
   float ph0[4];
 #pragma unroll
@@ -124,12 +169,6 @@ __device__ __forceinline__ void grad_cell3(const BlockDev& B, const NumDev& nd, 
   G[12] = mu + mut;
   G[13] = (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t));
   G[14] = ducros<float>(g);
-#pragma unroll
-  for (int x = 0; x < 3; ++x) {
-    Slo[x] = S0lo[x];
-    Slo[3 + x] = S1lo[x];
-    Slo[6 + x] = S2lo[x];
-  }
 }
 
 }  // namespace
@@ -146,7 +185,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
   __shared__ float sPr[4][NP2];                  // (rho, p', mu, e') of the current plane
   __shared__ float sG[NGW][NG2S];                // gradient records of the current plane
   __shared__ double sX[3][NG2S];                 // centroids of the current plane
-  __shared__ float sS[3][3][NG2S];               // lower i / j / k face vectors of the gradient cells
+  __shared__ float sS[3][3][NSS];                // face-vector slots (lower i / j / k faces)
   __shared__ float sFj[NFX][NT];                 // slot tile_tid(r, c)
   __shared__ float sFk[NFX][NT + C2J + 1];       // slot kface_slot(r, c)
   __shared__ unsigned short sUq[NT / 64][2 * 64];  // per wavefront: j / k faces with upwind work
@@ -245,6 +284,24 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
     }
   }
   ring_load(I0, I0 - 1);
+  // first gradient cell of each thread: region cell tid at owned planes, the own cell
+  // (tj + 1, tk + 1) at the two end planes; indices clamped into the block
+  auto cell_of = [&](int r, int& gj, int& gk, int& JJ, int& KK) {
+    gj = r / G2K; gk = r - gj * G2K;
+    JJ = ng + j0 - 1 + gj; KK = ng + k0 - 1 + gk;
+    const bool ok = g_ok(JJ, KK);
+    JJ = min(max(JJ, 0), N[1] - 1); KK = min(max(KK, 0), N[2] - 1);
+    return ok;
+  };
+  auto first_cell = [&](bool at_inner) { return at_inner ? tid : (tj + 1) * G2K + (tk + 1); };
+  GeoHi qn;                                      // upper i face and volume of the next plane's first cell
+  auto next_geometry = [&](int P) {              // plane P's face-vector slots and qn
+    face_dma(B, P, j0, k0, sS, tid);
+    int gj, gk, JJ, KK;
+    cell_of(first_cell(P >= I0 && P < Iend), gj, gk, JJ, KK);
+    geo_hi_load(B, P, JJ, KK, qn);
+  };
+  next_geometry(I0 - 1);
   float Gp[NGW];                                 // gradient record of the own cell at I-1
   // partial outward sum of the own cell I-1 in FP64: its lower i face minus its j / k face
   // sums (from the plane's exchange); the upper i face is subtracted when it is evaluated
@@ -276,31 +333,37 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
 #pragma unroll
       for (int x = 0; x < 3; ++x) sX[x][ring_s] = ring_x(x);
     }
+    // this plane's face-vector slots were loaded straight into LDS during the previous
+    // plane: every wavefront waits for its vector memory operations to complete (vmcnt(0);
+    // the barrier itself waits for LDS operations only) before the staging barrier
+    __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);
     __syncthreads();
     // ---- gradient records of plane I: the tile + ring (306 cells) at owned planes, the own
-    //      cells at the two end planes
+    //      cells at the two end planes.  The first cell's upper i face and volume were loaded
+    //      during the previous plane; wavefront 0 also takes the region's last 50 cells.
+    //      Cells outside the block's cells and first ghost layer are evaluated on clamped
+    //      indices and not stored.
     {
-      int rr[2] = {-1, -1};
-      if (inner) {
-        rr[0] = tid;
-        rr[1] = tid + NT < NG2 ? tid + NT : -1;
-      } else if (own) {
-        rr[0] = (tj + 1) * G2K + (tk + 1);
-      }
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int r = rr[u];
-        if (r < 0) continue;
-        const int gj = r / G2K, gk = r - gj * G2K;
-        const int JJ = ng + j0 - 1 + gj, KK = ng + k0 - 1 + gk;
-        if (!g_ok(JJ, KK)) continue;
-        float Gv[NGW], Slo[9];
-        grad_cell3(B, nd, sPh[s_m], sPh[s_0], sPh[s_p], sPr, (gj + 1) * H2K + (gk + 1), I, JJ, KK, Gv, Slo);
+      auto eval_store = [&](const GeoHi& q, int gj, int gk, bool ok) {
+        float Gv[NGW];
         const int gsl = gj * G2P + gk;
+        grad_cell3(nd, q, sS, gsl, sPh[s_m], sPh[s_0], sPh[s_p], sPr, (gj + 1) * H2K + (gk + 1), Gv);
+        if (ok) {
 #pragma unroll
-        for (int v = 0; v < NGW; ++v) sG[v][gsl] = Gv[v];
-#pragma unroll
-        for (int v = 0; v < 9; ++v) sS[v / 3][v % 3][gsl] = Slo[v];
+          for (int v = 0; v < NGW; ++v) sG[v][gsl] = Gv[v];
+        }
+      };
+      if (inner || own) {
+        int gj0, gk0, JJ0, KK0;
+        const bool ok0 = cell_of(first_cell(inner), gj0, gk0, JJ0, KK0);
+        eval_store(qn, gj0, gk0, ok0);
+      }
+      if (inner && tid + NT < NG2) {
+        int gj1, gk1, JJ1, KK1;
+        const bool ok1 = cell_of(tid + NT, gj1, gk1, JJ1, KK1);
+        GeoHi q1;
+        geo_hi_load(B, I, JJ1, KK1, q1);
+        eval_store(q1, gj1, gk1, ok1);
       }
     }
     __syncthreads();
@@ -482,6 +545,9 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       ring_load(I + 2, I + 1);
     }
     if (inner) __syncthreads();                  // (uniform) j / k face exchange
+    // the face-vector slots are free (this plane's face tasks are done): the next plane's
+    // go out now, and the epilogue covers their latency
+    if (I < Iend) next_geometry(I + 1);
     if (fin) {
       // stages 2 and 3: the base-state loads go out first, so the sums below overlap their
       // latency (before the barrier their registers would spill)
