@@ -24,6 +24,9 @@ constexpr int GJ = TJ + 2, GK = TK + 2;      // gradient / centroid region (halo
 constexpr int NPW = 7;                       // r u v w p e T
 constexpr int NGW = 15;                      // g[12], mu_eff, k_eff, theta
 constexpr int NFX = 6;                       // FP32-bulk face exchange: F'[5], w
+// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for (gfx9 encoding: vmcnt
+// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr unsigned WAITCNT_VM0 = 0x0F70;
 
 template <class R>
 struct ColT {                                // one cell's primitive record + T

@@ -56,9 +56,6 @@ __device__ __forceinline__ CellWT<float> rec_w(const Rec32& a) {
 // writes slot 64 w' + l of the instruction's run), issued a plane ahead.  NSS covers the
 // runs of the last wavefront that loads (360 slots used).
 constexpr int NSS_USED = (G2J + 1) * G2P, NSS = 384;
-// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for (gfx9 encoding: vmcnt
-// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
-constexpr unsigned WAITCNT_VM0 = 0x0F70;
 static_assert(NSS_USED <= NSS && NSS % 64 == 0 && NSS - NT <= 128, "face-vector slots");
 
 // Upper i face vector and FP32 volume of a gradient cell (the part of its geometry that is
