@@ -312,7 +312,7 @@ def bench_ensemble(args, wmx):
     if args.ens_forms == "eager":                # (no autotuner launches: the plan's own tiling)
         best, timings = ens.tiling()[:2], {}
     else:
-        best, timings = autotune.tune(ens, Qs0, steps=3, use_cache=False)
+        best, timings = autotune.tune(ens, Qs0, use_cache=False)
 
     def ens_step(Qs):
         return ens.step(Qs, graph=args.graph)[0]

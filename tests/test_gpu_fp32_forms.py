@@ -188,11 +188,9 @@ def test_fp32_ramp_steps_close_to_fp64():
 @pytest.mark.parametrize("kind", ["channel", "rot_cyc", "ramp"])
 def test_fused_gradients_match_the_separate_gradient_kernel(kind):
     """The fused FP32-bulk kernel (gradient records evaluated in the flux march) against the
-    separate gradient kernel + flux kernel: the same FP32 arithmetic per gradient record (the
-    fused kernel adds a blended face's upwind part after its viscous part, the separate one
-    before it), so the RHS agrees to FP32 rounding (bar 2e-6, max-normalised per component;
-    measured 4.6e-7) and one step to 5e-8 relative (the RHS difference times dt R / q;
-    measured 2.8e-8)."""
+    separate gradient kernel + flux kernel: the same FP32 arithmetic per gradient record, so
+    the RHS agrees to FP32 rounding of the records (bar 2e-6, max-normalised per component;
+    measured 4e-7) and one step to 2e-8 relative (the RHS difference times dt R / q)."""
     if kind == "channel":
         spec, arrays = cases.channel_spec(2, 48, 32, 16, {"kind": "Hybrid", "central": {"kind": "KEP", "order": 4}},
                                           "vreman", Re_b=2e4, splits=[0, 20, 48])
@@ -222,4 +220,4 @@ def test_fused_gradients_match_the_separate_gradient_kernel(kind):
     for a, b in zip(out[True][1], out[False][1]):
         err = float(np.abs(a - b).max() / np.abs(b).max())
         print(kind, "fused vs separate step", err)
-        assert err <= 5e-8, err
+        assert err <= 2e-8, err

@@ -26,8 +26,11 @@ def _key(obj):
     return f"{torch.cuda.get_device_name()}|{solvers[0].precision}|{scheme}|{dims}"
 
 
-def tune(obj, Q, steps=3, candidates=CANDIDATES, use_cache=True):
-    """Pick the fastest tiling for `obj` (Solver or EnsembleSolver) stepping state Q.
+def tune(obj, Q, steps=4, candidates=CANDIDATES, use_cache=True, rounds=2, warmup=10):
+    """Pick the fastest tiling for `obj` (Solver or EnsembleSolver) stepping state Q: after
+    `warmup` untimed steps (clocks and caches settle), every candidate is timed over
+    `steps` steps in each of `rounds` rounds (candidate order reversed in every other
+    round), and keeps its best round.  Q is not changed.
     Returns ((ci_len, order), {candidate: ms per flux launch})."""
     path = _cache_path()
     key = _key(obj)
@@ -42,27 +45,35 @@ def tune(obj, Q, steps=3, candidates=CANDIDATES, use_cache=True):
         obj.set_tiling(ci, order)
         return (ci, order), cache[key]["timings"]
     timings = {}
-    for ci, order in candidates:
-        obj.set_tiling(ci, order)
-        Qw = Q
-        Qw, _ = obj.step(Qw, graph=False)            # warm-up
-        torch.cuda.synchronize()
-        prof = obj if hasattr(obj, "profile") else None
-        if prof is not None:
-            prof.profile(("flux",))
-        t0 = torch.cuda.Event(enable_timing=True)
-        t1 = torch.cuda.Event(enable_timing=True)
-        t0.record()
-        for _ in range(steps):
-            Qw, _ = obj.step(Qw, graph=False)
-        t1.record()
-        torch.cuda.synchronize()
-        if prof is not None:
-            ms, n = prof.profile_read().get("flux", (0.0, 0))
-            prof.profile(())
-            timings[f"{ci},{order}"] = ms / max(n, 1)
-        else:
-            timings[f"{ci},{order}"] = t0.elapsed_time(t1) / steps
+    obj.set_tiling(*candidates[0])
+    Qw = Q
+    for _ in range(warmup):
+        Qw, _ = obj.step(Qw, graph=False)
+    torch.cuda.synchronize()
+    for rnd in range(rounds):
+        for ci, order in (candidates if rnd % 2 == 0 else candidates[::-1]):
+            obj.set_tiling(ci, order)
+            Qw = Q
+            Qw, _ = obj.step(Qw, graph=False)        # the candidate's first launches
+            torch.cuda.synchronize()
+            prof = obj if hasattr(obj, "profile") else None
+            if prof is not None:
+                prof.profile(("flux",))
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(steps):
+                Qw, _ = obj.step(Qw, graph=False)
+            t1.record()
+            torch.cuda.synchronize()
+            if prof is not None:
+                ms, n = prof.profile_read().get("flux", (0.0, 0))
+                prof.profile(())
+                t = ms / max(n, 1)
+            else:
+                t = t0.elapsed_time(t1) / steps
+            k = f"{ci},{order}"
+            timings[k] = min(timings.get(k, t), t)
     best = min(timings, key=timings.get)
     ci, order = (int(v) for v in best.split(","))
     obj.set_tiling(ci, order)

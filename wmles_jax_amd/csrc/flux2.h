@@ -24,9 +24,6 @@ constexpr int GJ = TJ + 2, GK = TK + 2;      // gradient / centroid region (halo
 constexpr int NPW = 7;                       // r u v w p e T
 constexpr int NGW = 15;                      // g[12], mu_eff, k_eff, theta
 constexpr int NFX = 6;                       // FP32-bulk face exchange: F'[5], w
-// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited for (gfx9 encoding: vmcnt
-// bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
-constexpr unsigned WAITCNT_VM0 = 0x0F70;
 
 template <class R>
 struct ColT {                                // one cell's primitive record + T
@@ -155,48 +152,21 @@ __device__ __forceinline__ void kep_pairs_f32(const CellWT<float>& L2, const Cel
   *un4 = (8.f * w - (wa + wb)) * (1.f / 12.f);
 }
 
-// Hybrid weight s of a face of scheme configuration SC (0: central only, 1: upwind only).
 template <int SC>
-__device__ __forceinline__ float face_weight(const NumDev& nd, float thL, float thR) {
-  const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
-                  : (SC == SC_KEP2 || SC == SC_KEP4) ? WMX_SCHEME_KEP : nd.scheme;
-  float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
-  if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
-  return s;
-}
-
-// Upwind part of a face with weight s > 0: Ff += s Fu in gauge form (momentum - p_ref S,
-// energy - e_ref Fu[0]), Fu the MUSCL-HLLC flux with the full pressure.
-template <int SC>
-__device__ __forceinline__ void upwind_add(const NumDev& nd, float s, const CellWT<float>& L2,
-                                           const CellWT<float>& L, const CellWT<float>& Rc,
-                                           const CellWT<float>& R2, float sx, float sy, float sz, float pr,
-                                           float er, float* Ff) {
-  const int limiter = SC == SC_HYB_KEP4_ALBADA ? 0 : nd.limiter;
-  float Fu[5];
-  muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz, pr, Fu);
-  Ff[0] = Ff[0] + s * Fu[0];
-  Ff[1] = Ff[1] + s * (Fu[1] - pr * sx);
-  Ff[2] = Ff[2] + s * (Fu[2] - pr * sy);
-  Ff[3] = Ff[3] + s * (Fu[3] - pr * sz);
-  Ff[4] = Ff[4] + s * (Fu[4] - er * Fu[0]);
-}
-
-// FP32-bulk face flux: (1 - s) central + s upwind + viscous, returns s.  DEFER: the upwind
-// part is left to the caller (upwind_add after the viscous part).
-template <int SC, bool DEFER = false>
-__device__ __forceinline__ float flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
-                                               const CellWT<float>& L, const CellWT<float>& Rc,
-                                               const CellWT<float>& R2, float thL, float thR,
-                                               const CellGT<float>& gL, const CellGT<float>& gR,
-                                               float TL, float TR, float d0, float d1, float d2,
-                                               float sx, float sy, float sz, float pr, float er,
-                                               float* Ff, float* uw) {
+__device__ __forceinline__ void flux_core_f32(const NumDev& nd, const CellWT<float>& L2,
+                                              const CellWT<float>& L, const CellWT<float>& Rc,
+                                              const CellWT<float>& R2, float thL, float thR,
+                                              const CellGT<float>& gL, const CellGT<float>& gR,
+                                              float TL, float TR, float d0, float d1, float d2,
+                                              float sx, float sy, float sz, float pr, float er,
+                                              float* Ff, float* uw) {
   const int sch = SC == SC_HYB_KEP4_ALBADA ? WMX_SCHEME_HYBRID
                   : (SC == SC_KEP2 || SC == SC_KEP4) ? WMX_SCHEME_KEP : nd.scheme;
   const int order = SC == SC_KEP2 ? 2 : ((SC == SC_HYB_KEP4_ALBADA || SC == SC_KEP4) ? 4 : nd.order);
-  const float s = face_weight<SC>(nd, thL, thR);
-  float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
+  const int limiter = SC == SC_HYB_KEP4_ALBADA ? 0 : nd.limiter;
+  float s = sch == WMX_SCHEME_MUSCLHLLC ? 1.f : 0.f;
+  if (sch == WMX_SCHEME_HYBRID) s = hybrid_weight(nd, thL, thR);
+  float Fc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, Fu[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, un4 = 0.f;
   if (s < 1.f) {
     const bool keep = SC == SC_ANY && (sch == WMX_SCHEME_KEEP ||
                                        (sch == WMX_SCHEME_HYBRID && nd.central == WMX_SCHEME_KEEP));
@@ -207,14 +177,20 @@ __device__ __forceinline__ float flux_core_f32(const NumDev& nd, const CellWT<fl
   const float wc = 1.f - s;
 #pragma unroll
   for (int v = 0; v < 5; ++v) Ff[v] = wc * Fc[v];
-  if (!DEFER && s > 0.f) upwind_add<SC>(nd, s, L2, L, Rc, R2, sx, sy, sz, pr, er, Ff);
+  if (s > 0.f) {
+    muscl_hllc_flux<float>(float(nd.gas.gamma), limiter, L2, L, Rc, R2, sx, sy, sz, pr, Fu);
+    Ff[0] = Ff[0] + s * Fu[0];
+    Ff[1] = Ff[1] + s * (Fu[1] - pr * sx);
+    Ff[2] = Ff[2] + s * (Fu[2] - pr * sy);
+    Ff[3] = Ff[3] + s * (Fu[3] - pr * sz);
+    Ff[4] = Ff[4] + s * (Fu[4] - er * Fu[0]);
+  }
   *uw = wc * un4;
   const float dist = fsqrt<float>((d0 * d0 + d1 * d1) + d2 * d2);
   const float id = fdiv<float>(1.f, dist);
   const float phL[4] = {L.u, L.v, L.w, TL};
   const float phR[4] = {Rc.u, Rc.v, Rc.w, TR};
   viscous_face_e<float>(gL, gR, phL, phR, d0 * id, d1 * id, d2 * id, id, sx, sy, sz, Ff);
-  return s;
 }
 
 template <class R>

@@ -38,87 +38,40 @@ __device__ __forceinline__ Rec32 load_rec32(const float* P, size_t cells, size_t
   return o;
 }
 
-// number of lanes below this one whose bit is set in the wavefront mask m
-__device__ __forceinline__ int lanes_below(unsigned long long m) {
-  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
 __device__ __forceinline__ CellWT<float> rec_w(const Rec32& a) {
   CellWT<float> w;
   w.r = a.r; w.u = a.u; w.v = a.v; w.w = a.w; w.p = a.p; w.e = a.e;
   return w;
 }
 
-// Face-vector slots of a tile in LDS: the lower i / j / k face vectors of the cells of the
-// gradient region's rows -1 .. 16 (18 rows of pitch G2P; slot gj * G2P + gk), so that the
-// upper j / k faces of the gradient cells are the lower faces of the slots one row / column
-// on.  They arrive by direct-to-LDS loads (global_load_lds_dword: lane l of a wavefront
-// writes slot 64 w' + l of the instruction's run), issued a plane ahead.  NSS covers the
-// runs of the last wavefront that loads (360 slots used).
-constexpr int NSS_USED = (G2J + 1) * G2P, NSS = 384;
-static_assert(NSS_USED <= NSS && NSS % 64 == 0 && NSS - NT <= 128, "face-vector slots");
-
-// Upper i face vector and FP32 volume of a gradient cell (the part of its geometry that is
-// not in the face-vector slots).
-struct GeoHi {
-  float S0hi[3], volr;
-};
-
-__device__ __forceinline__ void geo_hi_load(const BlockDev& B, int I, int J, int K, GeoHi& q) {
-  const unsigned st0 = (unsigned)B.N[1] * B.N[2];
-  const unsigned fc0 = (unsigned)fcount(B.N, 0);
-  const unsigned c = (unsigned)I * st0 + (unsigned)J * B.N[2] + (unsigned)K;
-#pragma unroll
-  for (int x = 0; x < 3; ++x) q.S0hi[x] = ldu(B.Sf[0] + x * (size_t)fc0, c + st0);
-  q.volr = ldu((const float*)B.volf, c);
-}
-
-// Direct-to-LDS loads of plane I's face-vector slots (slot s -> cell (s / G2P, s % G2P) of
-// the region, indices clamped into the block: slots that no gradient cell reads may hold
-// any cell's values).  The data lands when the loads complete; the workgroup barrier that
-// follows them waits for that (vmcnt) before any wavefront reads the slots.
-__device__ __forceinline__ void face_dma(const BlockDev& B, int I, int j0, int k0, float (*sS)[3][NSS], int tid) {
-  const int ng = B.ng, N1 = B.N[1], N2 = B.N[2];
+// Gradient record of cell (I, J, K) (P-region slot ps, gradient-region slot gs) from the
+// staged planes: the per-cell arithmetic of k_grad32 (grad32.h), faces in the order i, j, k.
+__device__ __forceinline__ void grad_cell3(const BlockDev& B, const NumDev& nd, const float (*pm)[NP2],
+                                           const float (*p0)[NP2], const float (*pp)[NP2],
+                                           const float (*pr)[NP2], int ps, int I, int J, int K, float* G,
+                                           float* Slo) {
+  const int N1 = B.N[1], N2 = B.N[2];
+  const unsigned st0 = (unsigned)N1 * N2;
   const unsigned fc0 = (unsigned)fcount(B.N, 0), fc1 = (unsigned)fcount(B.N, 1), fc2 = (unsigned)fcount(B.N, 2);
-#pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int s = tid + b * NT;
-    if (b == 1 && tid >= NSS - NT) break;                        // (uniform for whole runs)
-    const int gj = s / G2P, gk = s - gj * G2P;
-    const int J = min(max(ng + j0 - 1 + gj, 0), N1 - 1), K = min(max(ng + k0 - 1 + gk, 0), N2 - 1);
-    const unsigned f0 = (unsigned)I * ((unsigned)N1 * N2) + (unsigned)J * N2 + (unsigned)K;
-    const unsigned f1 = (unsigned)I * ((unsigned)(N1 + 1) * N2) + (unsigned)J * N2 + (unsigned)K;
-    const unsigned f2 = (unsigned)I * ((unsigned)N1 * (N2 + 1)) + (unsigned)J * (unsigned)(N2 + 1) + (unsigned)K;
-    const int run = s & ~63;                                     // the wavefront's 64-slot run
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      __builtin_amdgcn_global_load_lds((const void*)(B.Sf[0] + x * (size_t)fc0 + f0),
-                                       (__attribute__((address_space(3))) void*)&sS[0][x][run], 4, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void*)(B.Sf[1] + x * (size_t)fc1 + f1),
-                                       (__attribute__((address_space(3))) void*)&sS[1][x][run], 4, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void*)(B.Sf[2] + x * (size_t)fc2 + f2),
-                                       (__attribute__((address_space(3))) void*)&sS[2][x][run], 4, 0, 0);
-    }
-  }
-}
-
-// Gradient record of a cell (P-region slot ps, face-vector slot gsl) from the staged planes,
-// the face-vector slots and q: the per-cell arithmetic of k_grad32 (grad32.h), faces in the
-// order i, j, k.
-__device__ __forceinline__ void grad_cell3(const NumDev& nd, const GeoHi& q, const float (*sS)[3][NSS], int gsl,
-                                           const float (*pm)[NP2], const float (*p0)[NP2], const float (*pp)[NP2],
-                                           const float (*pr)[NP2], int ps, float* G) {
+  const unsigned c = (unsigned)I * st0 + (unsigned)J * N2 + (unsigned)K;
+  const unsigned f0 = c;
+  const unsigned f1 = (unsigned)I * ((unsigned)(N1 + 1) * N2) + (unsigned)J * N2 + (unsigned)K;
+  const unsigned f2 = (unsigned)I * ((unsigned)N1 * (N2 + 1)) + (unsigned)J * (unsigned)(N2 + 1) + (unsigned)K;
+  const float* S0 = B.Sf[0];
+  const float* S1 = B.Sf[1];
+  const float* S2 = B.Sf[2];
+  // every load first (one memory round trip), then the arithmetic
   float S0lo[3], S0hi[3], S1lo[3], S1hi[3], S2lo[3], S2hi[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    S0lo[x] = sS[0][x][gsl];
-    S0hi[x] = q.S0hi[x];
-    S1lo[x] = sS[1][x][gsl];
-    S1hi[x] = sS[1][x][gsl + G2P];
-    S2lo[x] = sS[2][x][gsl];
-    S2hi[x] = sS[2][x][gsl + 1];
+    S0lo[x] = ldu(S0 + x * (size_t)fc0, f0);
+    S0hi[x] = ldu(S0 + x * (size_t)fc0, f0 + st0);
+    S1lo[x] = ldu(S1 + x * (size_t)fc1, f1);
+    S1hi[x] = ldu(S1 + x * (size_t)fc1, f1 + (unsigned)N2);
+    S2lo[x] = ldu(S2 + x * (size_t)fc2, f2);
+    S2hi[x] = ldu(S2 + x * (size_t)fc2, f2 + 1);
   }
-  const float volr = q.volr;
+  const float volr = ldu((const float*)B.volf, c);
   const float h = 0.5f;
   float ph0[4];
 #pragma unroll
@@ -166,6 +119,12 @@ __device__ __forceinline__ void grad_cell3(const NumDev& nd, const GeoHi& q, con
   G[12] = mu + mut;
   G[13] = (float)gs.cp * (fdiv<float>(mu, (float)gs.Pr) + fdiv<float>(mut, (float)gs.Pr_t));
   G[14] = ducros<float>(g);
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    Slo[x] = S0lo[x];
+    Slo[3 + x] = S1lo[x];
+    Slo[6 + x] = S2lo[x];
+  }
 }
 
 }  // namespace
@@ -182,17 +141,15 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
   __shared__ float sPr[4][NP2];                  // (rho, p', mu, e') of the current plane
   __shared__ float sG[NGW][NG2S];                // gradient records of the current plane
   __shared__ double sX[3][NG2S];                 // centroids of the current plane
-  __shared__ float sS[3][3][NSS];                // face-vector slots (lower i / j / k faces)
+  __shared__ float sS[3][3][NG2S];               // lower i / j / k face vectors of the gradient cells
   __shared__ float sFj[NFX][NT];                 // slot tile_tid(r, c)
   __shared__ float sFk[NFX][NT + C2J + 1];       // slot kface_slot(r, c)
-  __shared__ unsigned short sUq[NT / 64][2 * 64];  // per wavefront: j / k faces with upwind work
   const int4 t = tiles[tile];
   const BlockDev& B = blks[t.x];
   const int j0 = t.y, k0 = t.z, i0 = t.w;
   const int i1 = min(i0 + ci_len, B.n[0]);
   const int tid = threadIdx.x;
   const int tj = (tid >> 6) + 4 * ((tid >> 4) & 3), tk = tid & 15;   // tile_tid(tj, tk) == tid
-  const int wv = tid >> 6, lane = tid & 63;
   const int ng = B.ng;
   const int* N = B.N;
   const int n0 = B.n[0], n1 = B.n[1], n2 = B.n[2];
@@ -281,24 +238,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
     }
   }
   ring_load(I0, I0 - 1);
-  // first gradient cell of each thread: region cell tid at owned planes, the own cell
-  // (tj + 1, tk + 1) at the two end planes; indices clamped into the block
-  auto cell_of = [&](int r, int& gj, int& gk, int& JJ, int& KK) {
-    gj = r / G2K; gk = r - gj * G2K;
-    JJ = ng + j0 - 1 + gj; KK = ng + k0 - 1 + gk;
-    const bool ok = g_ok(JJ, KK);
-    JJ = min(max(JJ, 0), N[1] - 1); KK = min(max(KK, 0), N[2] - 1);
-    return ok;
-  };
-  auto first_cell = [&](bool at_inner) { return at_inner ? tid : (tj + 1) * G2K + (tk + 1); };
-  GeoHi qn;                                      // upper i face and volume of the next plane's first cell
-  auto next_geometry = [&](int P) {              // plane P's face-vector slots and qn
-    face_dma(B, P, j0, k0, sS, tid);
-    int gj, gk, JJ, KK;
-    cell_of(first_cell(P >= I0 && P < Iend), gj, gk, JJ, KK);
-    geo_hi_load(B, P, JJ, KK, qn);
-  };
-  next_geometry(I0 - 1);
   float Gp[NGW];                                 // gradient record of the own cell at I-1
   // partial outward sum of the own cell I-1 in FP64: its lower i face minus its j / k face
   // sums (from the plane's exchange); the upper i face is subtracted when it is evaluated
@@ -330,37 +269,31 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
 #pragma unroll
       for (int x = 0; x < 3; ++x) sX[x][ring_s] = ring_x(x);
     }
-    // this plane's face-vector slots were loaded straight into LDS during the previous
-    // plane: every wavefront waits for its vector memory operations to complete (vmcnt(0);
-    // the barrier itself waits for LDS operations only) before the staging barrier
-    __builtin_amdgcn_s_waitcnt(WAITCNT_VM0);
     __syncthreads();
     // ---- gradient records of plane I: the tile + ring (306 cells) at owned planes, the own
-    //      cells at the two end planes.  The first cell's upper i face and volume were loaded
-    //      during the previous plane; wavefront 0 also takes the region's last 50 cells.
-    //      Cells outside the block's cells and first ghost layer are evaluated on clamped
-    //      indices and not stored.
+    //      cells at the two end planes
     {
-      auto eval_store = [&](const GeoHi& q, int gj, int gk, bool ok) {
-        float Gv[NGW];
-        const int gsl = gj * G2P + gk;
-        grad_cell3(nd, q, sS, gsl, sPh[s_m], sPh[s_0], sPh[s_p], sPr, (gj + 1) * H2K + (gk + 1), Gv);
-        if (ok) {
-#pragma unroll
-          for (int v = 0; v < NGW; ++v) sG[v][gsl] = Gv[v];
-        }
-      };
-      if (inner || own) {
-        int gj0, gk0, JJ0, KK0;
-        const bool ok0 = cell_of(first_cell(inner), gj0, gk0, JJ0, KK0);
-        eval_store(qn, gj0, gk0, ok0);
+      int rr[2] = {-1, -1};
+      if (inner) {
+        rr[0] = tid;
+        rr[1] = tid + NT < NG2 ? tid + NT : -1;
+      } else if (own) {
+        rr[0] = (tj + 1) * G2K + (tk + 1);
       }
-      if (inner && tid + NT < NG2) {
-        int gj1, gk1, JJ1, KK1;
-        const bool ok1 = cell_of(tid + NT, gj1, gk1, JJ1, KK1);
-        GeoHi q1;
-        geo_hi_load(B, I, JJ1, KK1, q1);
-        eval_store(q1, gj1, gk1, ok1);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int r = rr[u];
+        if (r < 0) continue;
+        const int gj = r / G2K, gk = r - gj * G2K;
+        const int JJ = ng + j0 - 1 + gj, KK = ng + k0 - 1 + gk;
+        if (!g_ok(JJ, KK)) continue;
+        float Gv[NGW], Slo[9];
+        grad_cell3(B, nd, sPh[s_m], sPh[s_0], sPh[s_p], sPr, (gj + 1) * H2K + (gk + 1), I, JJ, KK, Gv, Slo);
+        const int gsl = gj * G2P + gk;
+#pragma unroll
+        for (int v = 0; v < NGW; ++v) sG[v][gsl] = Gv[v];
+#pragma unroll
+        for (int v = 0; v < 9; ++v) sS[v / 3][v % 3][gsl] = Slo[v];
       }
     }
     __syncthreads();
@@ -378,16 +311,9 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       for (int v = 0; v < 5; ++v) vq[v] = ldu(qi + v * cells, c);
     }
     float Fin[NFX] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    // Hybrid configurations: the tasks evaluate the central and viscous parts; the upwind
-    // (MUSCL-HLLC) parts of the faces with s > 0 are evaluated afterwards in packed passes
-    // of the wavefront (the j / k faces listed in sUq, the own i face flagged in up_i), so
-    // that a wavefront runs the upwind arithmetic once per plane instead of once per task
-    constexpr bool DEFER = SC == SC_HYB_KEP4_ALBADA || SC == SC_ANY;
-    int nq = 0;                                  // listed faces of the wavefront (uniform)
-    bool up_i = false;
 #pragma unroll 1
     for (int task = inner ? 0 : 2; task < (I >= I0 ? 3 : 0); ++task) {
-      bool doit = act, later = false;
+      bool doit = act;
       int d = 0, fI = I, fJ = J, fK = K;
       int rj = tj, rk = tk;
       CellWT<float> L2, L, Rc, R2;
@@ -453,10 +379,8 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         } else {
           // the face vector, staged by the gradient cell whose lower face it is
           const int ss = task == 2 ? sg_own : (rj + 1) * G2P + (rk + 1);
-          const float sw = flux_core_f32<SC, DEFER>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, dx[0], dx[1], dx[2],
-                                                    sS[d][0][ss], sS[d][1][ss], sS[d][2][ss], float(B.p_ref),
-                                                    float(B.e_ref), F, &F[5]);
-          later = DEFER && sw > 0.f;
+          flux_core_f32<SC>(nd, L2, L, Rc, R2, thL, thR, gL, gR, TL, TR, dx[0], dx[1], dx[2], sS[d][0][ss], sS[d][1][ss],
+                            sS[d][2][ss], float(B.p_ref), float(B.e_ref), F, &F[5]);
         }
         if (task == 2) {
 #pragma unroll
@@ -469,68 +393,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
           for (int v = 0; v < NFX; ++v) sFk[v][kface_slot(rj, rk)] = F[v];
         }
       }
-      if (DEFER) {
-        if (task == 2) {
-          up_i = later;
-        } else {
-          const unsigned long long m = __ballot(later);
-          if (later) sUq[wv][nq + lanes_below(m)] = (unsigned short)((d << 10) | (rj << 5) | rk);
-          nq += __popcll(m);
-        }
-      }
-    }
-    if (DEFER) {
-      // packed upwind passes: in the first, the lanes whose i face has upwind work take it
-      // and the other lanes the first listed faces; then 64 listed faces per pass
-      const unsigned long long mi = __ballot(up_i);
-      const int ni = __popcll(mi), rank_free = lanes_below(~mi);
-      const float pr = float(B.p_ref), er = float(B.e_ref);
-      for (int qb = 0, pass = 0; (pass == 0 && ni > 0) || qb < nq; ++pass) {
-        const bool mine = pass == 0 && up_i;
-        const int q = qb + (pass == 0 ? rank_free : lane);
-        const bool lst = !mine && q < nq;
-        if (mine || lst) {
-          CellWT<float> L2, L, Rc, R2;
-          float thL, thR, sv[3];
-          int dd = 0, rj = 0, rk = 0;
-          if (mine) {
-            L2 = rec_w(Pw[0]); L = rec_w(Pw[1]); Rc = rec_w(Pw[2]); R2 = rec_w(Pw[3]);
-            thL = Gp[14]; thR = sG[14][sg_own];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) sv[x] = sS[0][x][sg_own];
-          } else {
-            const int dsc = sUq[wv][q];
-            dd = dsc >> 10; rj = (dsc >> 5) & 31; rk = dsc & 31;
-            const int pR = (rj + 2) * H2K + (rk + 2), gR_ = (rj + 1) * G2P + (rk + 1);
-            const int pst = dd == 1 ? H2K : 1, gst = dd == 1 ? G2P : 1;
-            auto cw = [&](int s) {
-              CellWT<float> w;
-              w.r = sPr[0][s]; w.u = sPh[s_0][0][s]; w.v = sPh[s_0][1][s]; w.w = sPh[s_0][2][s];
-              w.p = sPr[1][s]; w.e = sPr[3][s];
-              return w;
-            };
-            L2 = cw(pR - 2 * pst); L = cw(pR - pst); Rc = cw(pR); R2 = cw(pR + pst);
-            thL = sG[14][gR_ - gst]; thR = sG[14][gR_];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) sv[x] = sS[dd][x][gR_];
-          }
-          float dF[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-          upwind_add<SC>(nd, face_weight<SC>(nd, thL, thR), L2, L, Rc, R2, sv[0], sv[1], sv[2], pr, er, dF);
-          if (mine) {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Fin[v] = Fin[v] + dF[v];
-          } else if (dd == 1) {
-            const int sl = tile_tid(rj, rk);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) sFj[v][sl] = sFj[v][sl] + dF[v];
-          } else {
-            const int sl = kface_slot(rj, rk);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) sFk[v][sl] = sFk[v][sl] + dF[v];
-          }
-        }
-        qb += pass == 0 ? 64 - ni : 64;
-      }
     }
     // ---- prefetch for plane I+1 (own record of I+2, own centroid of I+1, the ring), issued
     //      before the exchange barrier so that the epilogue and the barrier cover its
@@ -542,9 +404,6 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       ring_load(I + 2, I + 1);
     }
     if (inner) __syncthreads();                  // (uniform) j / k face exchange
-    // the face-vector slots are free (this plane's face tasks are done): the next plane's
-    // go out now, and the epilogue covers their latency
-    if (I < Iend) next_geometry(I + 1);
     if (fin) {
       // stages 2 and 3: the base-state loads go out first, so the sums below overlap their
       // latency (before the barrier their registers would spill)
