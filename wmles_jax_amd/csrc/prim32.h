@@ -192,18 +192,30 @@ __device__ __forceinline__ void sponge_head(const BlockDev* __restrict__ blks, i
 }
 
 // ------------------------------------------------------------------------------ prim
-// Primitive records of every cell (ghosts included).  DT: the stage-1 launch of a step
+// Primitive records of the cells that are read (rec_read: ghosts up to 2 layers deep).  DT:
+// the stage-1 launch of a step
 // also takes the CFL partial of its workgroup's interior cells into part[workgroup],
 // reusing the state it loads (the per-case min then follows as for k_dt_partial).
-__device__ __forceinline__ bool interior_of(const BlockDev& B, unsigned c, int* I, int* J, int* K) {
+__device__ __forceinline__ void cell_ijk(const BlockDev& B, unsigned c, int* I, int* J, int* K) {
   const unsigned N1 = (unsigned)B.N[1], N2 = (unsigned)B.N[2];
   const unsigned r = c / N2;
   *K = (int)(c - r * N2);
   *J = (int)(r % N1);
   *I = (int)(r / N1);
+}
+__device__ __forceinline__ bool interior_of(const BlockDev& B, unsigned c, int* I, int* J, int* K) {
+  cell_ijk(B, c, I, J, K);
   const int ng = B.ng;
   return *I >= ng && *I < ng + B.n[0] && *J >= ng && *J < ng + B.n[1] && *K >= ng &&
          *K < ng + B.n[2];
+}
+// Cells whose primitive record some kernel reads: the block's cells and the two ghost
+// layers next to them (the face stencils and the gradient records reach 2 cells past the
+// block's cells; the gradient cells, 1).  The outer ghost layers (ng > 2) keep their
+// state, but no record is formed for them.
+__device__ __forceinline__ bool rec_read(const BlockDev& B, int I, int J, int K) {
+  const int lo = B.ng - 2;
+  return I >= lo && I < B.N[0] - lo && J >= lo && J < B.N[1] - lo && K >= lo && K < B.N[2] - lo;
 }
 
 // State of cell c: from q, or (zfuse) for a k ghost of a flagged side the value the sweep's
@@ -260,12 +272,13 @@ __device__ __forceinline__ void prim32_block(const BlockDev* __restrict__ blks,
   const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
   const long long cells = B.cells;
   double dtc = __builtin_huge_val();
-  if (c < cells) {
+  int I = 0, J = 0, K = 0;
+  if (c < cells) cell_ijk(B, (unsigned)c, &I, &J, &K);
+  if (c < cells && rec_read(B, I, J, K)) {
     const GasC& gas = nd.gas;
     double qv[5];
     prim_state(blks, b, c, q, gas, zfuse, qv);
     rec32_store(B.Pf, (size_t)cells, (size_t)c, nd, qv, B.p_ref, B.T_gauge, B.e_ref);
-    int I, J, K;
     if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) {
       const Prim w = cons_to_prim(gas, qv[0], qv[1], qv[2], qv[3], qv[4]);
       dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);

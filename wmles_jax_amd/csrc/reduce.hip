@@ -56,7 +56,9 @@ __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
   const long long c = (long long)(wg - wg_prefix[b]) * blockDim.x + threadIdx.x;
   const long long cells = B.cells;
   double dtc = __builtin_huge_val();
-  if (c < cells) {
+  int I = 0, J = 0, K = 0;
+  if (c < cells) cell_ijk(B, (unsigned)c, &I, &J, &K);
+  if (c < cells && rec_read(B, I, J, K)) {
     const GasC& gas = nd.gas;
     double qv[5];
     prim_state(blks, b, c, q, gas, zfuse, qv);
@@ -70,7 +72,6 @@ __global__ __launch_bounds__(256) void k_prim(const BlockDev* __restrict__ blks,
     P[5 * cells + c] = w.T;
     P[6 * cells + c] = mu_suth(gas, w.T);
     P[7 * cells + c] = w.p / ((gas.gamma - 1.0) * w.r);
-    int I, J, K;
     if (DT && interior_of(B, (unsigned)c, &I, &J, &K)) dtc = cell_dt(B, nd, (unsigned)c, I, J, K, w);
   }
   if (DT) {
