@@ -18,12 +18,21 @@
 // The march runs over planes I0-1 .. I0+len, one plane more at each end than the tile's
 // cells.  Three barriers per plane (staging, gradients, face exchange; the last is skipped
 // at the end planes, which have no j / k faces).
+//
+// Wavefront priorities (s_setprio) by phase: two workgroups share each CU, one wavefront of
+// each per SIMD, and the launch is bound by latency at the barriers, not by issue.  A
+// wavefront in its gradient phase issues first (its workgroup's next barrier waits for the
+// slowest wavefront's gradient cells, whose geometry loads carry a memory round trip), then
+// one in its face tasks; the epilogue and the staging run at the base priority.
 #pragma once
 #include "flux2.h"
 
 namespace wmx {
 
 namespace {
+
+// wavefront priorities of the phases of a plane (s_setprio operands, 0..3)
+constexpr int PRIO_GRAD = 2, PRIO_FACE = 1, PRIO_REST = 0;
 
 // a cell's FP32-bulk primitive record incl. mu: groups (u, v, w, T') and (rho, p', mu, e')
 struct Rec32 {
@@ -270,6 +279,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       for (int x = 0; x < 3; ++x) sX[x][ring_s] = ring_x(x);
     }
     __syncthreads();
+    __builtin_amdgcn_s_setprio(PRIO_GRAD);
     // ---- gradient records of plane I: the tile + ring (306 cells) at owned planes, the own
     //      cells at the two end planes
     {
@@ -296,6 +306,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
         for (int v = 0; v < 9; ++v) sS[v / 3][v % 3][gsl] = Slo[v];
       }
     }
+    __builtin_amdgcn_s_setprio(PRIO_FACE);
     __syncthreads();
     // ---- face fluxes: 0 j face below row tj, 1 k face left of column tk (or the upper
     //      edge k face of row tk), 2 the own i face I-1/2
@@ -404,6 +415,7 @@ __device__ __forceinline__ void flux3_tile(const BlockDev* __restrict__ blks, co
       ring_load(I + 2, I + 1);
     }
     if (inner) __syncthreads();                  // (uniform) j / k face exchange
+    __builtin_amdgcn_s_setprio(PRIO_REST);
     if (fin) {
       // stages 2 and 3: the base-state loads go out first, so the sums below overlap their
       // latency (before the barrier their registers would spill)
