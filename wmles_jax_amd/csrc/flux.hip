@@ -177,9 +177,10 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
   const int pl[NPW64] = {P_R, P_U, P_V, P_W, P_P, P_E, P_T};
   for (int i = i0; i < i1; ++i) {
     const int I = ng + i;
-    // the staging phase (loads whose round trip the barrier below waits for) at a raised
-    // wavefront priority: of the two workgroups on a CU, the one staging issues first
-    __builtin_amdgcn_s_setprio(1);
+    // wavefront priorities by phase: the staging (loads whose round trip the barrier below
+    // waits for) 2, the face tasks 1, the epilogue 0; of the two workgroups on a CU, the
+    // one nearer its next barrier issues first
+    __builtin_amdgcn_s_setprio(2);
     // plane I staged for the j / k faces and the lower cell of the i face: the gradient
     // records on the tile + 1-cell ring (rows j0-1 .. j0+15, columns k0-1 .. k0+16, within
     // the cells the gradient launch covers) and the primitives on the tile + 2-cell ring
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
         }
       }
     }
-    __builtin_amdgcn_s_setprio(0);
+    __builtin_amdgcn_s_setprio(1);
     __syncthreads();
 #pragma unroll 1
     for (int task = 0; task < 3; ++task) {
@@ -253,6 +254,7 @@ __global__ __launch_bounds__(256) void k_flux1(const BlockDev* __restrict__ blks
         }
       }
     }
+    __builtin_amdgcn_s_setprio(0);
     // the epilogue's state and volume loads go out before the barrier
     const long long c = cidx(B.N, I, J, K);
     const gconst<double> qi = gmem(qin.p[t.x]);
