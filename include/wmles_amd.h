@@ -246,8 +246,8 @@ int wmx_plan_set_tiling(wmx_plan* plan, int32_t ci_len, int32_t order);
    evaluates the Green-Gauss gradients, SGS viscosity and Ducros sensor of each plane in
    LDS as it marches, so no gradient records go through HBM and the separate gradient launch
    keeps only its side work (per-case dt min, Newton wall faces); on == 0 runs the separate
-   gradient kernel and the flux kernel that reads its records.  Ignored by FP64 plans and
-   by persistent stages (which run the separate form). */
+   gradient kernel and the flux kernel that reads its records.  Persistent stages follow
+   the same choice; FP64 plans ignore it. */
 int wmx_plan_set_fused_gradients(wmx_plan* plan, int32_t on);
 /* Persistent stages (the batching form's launch structure; no reference counterpart):
    on != 0 runs every RK stage of wmx_advance / wmx_step as ONE cooperative launch -- a
