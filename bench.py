@@ -26,7 +26,8 @@ Other workloads (one JSON line each):
               here on the GPU) against the Newton model: wall-kernel time, u_tau error
 
 Printed (one JSON line): Mcell-updates/s, ms/step, the roofline of the dominant
-kernel (fused flux / divergence / stage update) from live HIP events, and the CPU
+kernel (fused flux / divergence / stage update) from live HIP events (against the
+datasheet HBM peak, and beside it the device's measured copy bandwidth), and the CPU
 baseline (the FP64 oracle timed on the host on a bounded sample of the same
 workload).
 """
@@ -181,6 +182,35 @@ FLUX_KERNEL = {True: "k_flux3 (gradient records in the march + face fluxes + div
                None: "k_flux1 (FP64: face fluxes + divergence + RK stage)"}
 
 
+_COPY_PEAK = []
+
+
+def copy_peak_gbs():
+    """Measured device copy bandwidth (the STREAM copy of this device): 2 GiB of doubles
+    copied with torch's vectorised element-wise copy, HIP events around each of 20 copies
+    after 5 untimed ones, the best copy; GB/s counts the bytes read plus written."""
+    import torch
+    if not _COPY_PEAK:
+        n = 256 * 1024 * 1024
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        best = float("inf")
+        for _ in range(20):
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            y.copy_(x)
+            t1.record()
+            t1.synchronize()
+            best = min(best, t0.elapsed_time(t1) * 1e-3)
+        del x, y
+        torch.cuda.empty_cache()
+        _COPY_PEAK.append(2 * n * 8 / best / 1e9)
+    return _COPY_PEAK[0]
+
+
 def _roofline(avg_launch_s, b_launch, workload, b_cell_step, fused=True):
     """achieved = algorithmic bytes per flux launch / the launch time measured live with
     HIP events on the launch stream.  The committed rocprofv3 trace's average launch
@@ -201,6 +231,10 @@ def _roofline(avg_launch_s, b_launch, workload, b_cell_step, fused=True):
     for k in ("valu_issue_busy", "valu_lane_utilisation"):
         if k in prof:
             rec["rocprof_" + k] = prof[k]
+    # the device's measured copy bandwidth beside the datasheet peak
+    cp = copy_peak_gbs()
+    rec["measured_copy_peak"] = round(cp, 1)
+    rec["frac_of_measured_copy_peak"] = round(achieved / cp, 4)
     return rec
 
 
