@@ -369,38 +369,40 @@ WMX_HD long long fcount(const int* N, int d) {
 }
 
 // ghost shell cell s -> (I, J, K): i-slabs (full j,k), j-slabs (interior i), k-slabs
-// (interior i, j); k fastest within each
-WMX_HD void shell_cell(long long s, const int* N, const int* n, int ng, int* I, int* J, int* K) {
-  long long a = 2LL * ng * N[1] * N[2];
+// (interior i, j); k fastest within each.  32-bit arithmetic: a plan's blocks hold fewer
+// than 2^29 cells (each FP64 plane stays below 4 GiB), and the shell is part of the block.
+WMX_HD void shell_cell(unsigned s, const int* N, const int* n, int ng, int* I, int* J, int* K) {
+  const unsigned N1 = (unsigned)N[1], N2 = (unsigned)N[2], ung = (unsigned)ng;
+  const unsigned a = 2u * ung * N1 * N2;
   if (s < a) {
-    long long l = s / ((long long)N[1] * N[2]);
-    long long r = s % ((long long)N[1] * N[2]);
-    *I = l < ng ? (int)l : (int)(n[0] + l);
-    *J = (int)(r / N[2]);
-    *K = (int)(r % N[2]);
+    const unsigned l = s / (N1 * N2), r = s - l * (N1 * N2);
+    const unsigned j = r / N2;
+    *I = l < ung ? (int)l : (int)(n[0] + l);
+    *J = (int)j;
+    *K = (int)(r - j * N2);
     return;
   }
   s -= a;
-  long long b = 2LL * ng * n[0] * N[2];
-  if (s < b) {
-    long long per = (long long)n[0] * N[2];
-    long long l = s / per;
-    long long r = s % per;
-    *I = ng + (int)(r / N[2]);
-    *K = (int)(r % N[2]);
-    *J = l < ng ? (int)l : (int)(n[1] + l);
+  const unsigned per_j = (unsigned)n[0] * N2;
+  if (s < 2u * ung * per_j) {
+    const unsigned l = s / per_j, r = s - l * per_j;
+    const unsigned i = r / N2;
+    *I = ng + (int)i;
+    *K = (int)(r - i * N2);
+    *J = l < ung ? (int)l : (int)(n[1] + l);
     return;
   }
-  s -= b;
+  s -= 2u * ung * per_j;
   // k-slabs: side, then interior (I, J) row, then the ng ghost layers of the row (contiguous
   // along k, so a wavefront touches ng-cell runs instead of one cell per row)
-  long long per = (long long)n[0] * n[1] * ng;
+  const unsigned per = (unsigned)n[0] * (unsigned)n[1] * ung;
   const int side = s < per ? 0 : 1;
-  long long t = s - side * per;
-  long long row = t / ng;
-  const int m = (int)(t - row * ng);
-  *I = ng + (int)(row / n[1]);
-  *J = ng + (int)(row % n[1]);
+  const unsigned t = s - (side ? per : 0u);
+  const unsigned row = t / ung;
+  const int m = (int)(t - row * ung);
+  const unsigned i = row / (unsigned)n[1];
+  *I = ng + (int)i;
+  *J = ng + (int)(row - i * (unsigned)n[1]);
   *K = side == 0 ? m : ng + n[2] + m;
 }
 

@@ -142,21 +142,30 @@ __global__ __launch_bounds__(256) void k_wall(const BlockDev* __restrict__ blks,
 }
 
 // ------------------------------------------------------------------- shells / sponge
+// Both shell kernels take every load of a cell before its first store: a store between
+// two loads would make each of the five components a separate memory round trip (the
+// compiler cannot move a load across a store that may alias it).
 // save: shell <- q ghosts
 __global__ void k_shell_save(const BlockDev* __restrict__ blks, const int* __restrict__ wg_prefix,
                              int nblk, Ptrs q) {
   const int b = find_block(wg_prefix, nblk, blockIdx.x);
   const BlockDev& B = blks[b];
-  long long s = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  if (s >= B.shell_cells) return;
+  const unsigned s = (blockIdx.x - (unsigned)wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const size_t nsh = (size_t)B.shell_cells, cells = (size_t)B.cells;
+  if (s >= nsh) return;
   int I, J, K;
   shell_cell(s, B.N, B.n, B.ng, &I, &J, &K);
   // (interior-(I, J) k ghosts of fused sides: q0 keeps them through the step)
   if (I >= B.ng && I < B.ng + B.n[0] && J >= B.ng && J < B.ng + B.n[1] &&
       shell_kept(B.n, B.ng, B.zfuse, K))
     return;
-  const long long c = cidx(B.N, I, J, K);
-  for (int v = 0; v < 5; ++v) B.shell[v * B.shell_cells + s] = q.p[b][v * B.cells + c];
+  const double* src = q.p[b] + cidx(B.N, I, J, K);
+  double* dst = B.shell + s;
+  double g[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) g[v] = src[v * cells];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) dst[v * nsh] = g[v];
 }
 
 // q0 ghosts <- shell (restore); out ghosts <- the three stage combinations applied to
@@ -167,13 +176,16 @@ __global__ void k_shell_finish(const BlockDev* __restrict__ blks, const int* __r
                                double dt_host) {
   const int b = find_block(wg_prefix, nblk, blockIdx.x);
   const BlockDev& B = blks[b];
-  long long s = (long long)(blockIdx.x - wg_prefix[b]) * blockDim.x + threadIdx.x;
-  if (s >= B.shell_cells) return;
+  const unsigned s = (blockIdx.x - (unsigned)wg_prefix[b]) * blockDim.x + threadIdx.x;
+  const size_t nsh = (size_t)B.shell_cells, cells = (size_t)B.cells;
+  if (s >= nsh) return;
   int I, J, K;
   shell_cell(s, B.N, B.n, B.ng, &I, &J, &K);
   const long long c = cidx(B.N, I, J, K);
+  double* oc = out.p[b] + c;
   if (MODE == 0) {
-    for (int v = 0; v < 5; ++v) out.p[b][v * B.cells + c] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) oc[v * cells] = 0.0;
     return;
   }
   const double dt = d_dt ? d_dt[B.case_id] : dt_host;
@@ -181,12 +193,23 @@ __global__ void k_shell_finish(const BlockDev* __restrict__ blks, const int* __r
   // interior-(I, J) k ghosts of fused sides were never written in q0: read there, no restore
   const bool kept = I >= B.ng && I < B.ng + B.n[0] && J >= B.ng && J < B.ng + B.n[1] &&
                     shell_kept(B.n, B.ng, B.zfuse, K);
+  double* qc = q0.p[b] + c;
+  double g[5];
+  if (kept) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) g[v] = qc[v * cells];
+  } else {
+    const double* sc = B.shell + s;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) g[v] = sc[v * nsh];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qc[v * cells] = g[v];
+  }
+#pragma unroll
   for (int v = 0; v < 5; ++v) {
-    const double g = kept ? q0.p[b][v * B.cells + c] : B.shell[v * B.shell_cells + s];
-    if (!kept) q0.p[b][v * B.cells + c] = g;
-    const double q1 = g + zr;
-    const double q2 = 0.75 * g + 0.25 * (q1 + zr);
-    out.p[b][v * B.cells + c] = (1.0 / 3.0) * g + (2.0 / 3.0) * (q2 + zr);
+    const double q1 = g[v] + zr;
+    const double q2 = 0.75 * g[v] + 0.25 * (q1 + zr);
+    oc[v * cells] = (1.0 / 3.0) * g[v] + (2.0 / 3.0) * (q2 + zr);
   }
 }
 
