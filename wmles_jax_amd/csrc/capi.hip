@@ -887,10 +887,22 @@ int wmx_plan_bind_workspace(wmx_plan* p, void* ws, size_t bytes) {
   p->d_stat = reinterpret_cast<double*>(p->ws + p->off_stat);
   WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
   // the CFL dt's cell-mean face vectors, made once: the geometry may still be in flight on
-  // the caller's stream, so the device is synchronised on both sides
+  // the caller's stream, so the device is synchronised on both sides.  The planes that hold
+  // +0.0 in every interior cell (axis-aligned directions) become BlockDev::sb_zero, whose
+  // loads the dt skips; the kernel ORs the non-zero planes into one word per block, kept in
+  // the partials buffer (at least two doubles per block, unused until the first step).
   WMX_CHECK(hipDeviceSynchronize());
-  for (int b = 0; b < p->nb; ++b) WMX_CHECK(launch_cell_sb(p->hb[b], nullptr));
+  unsigned* d_nz = reinterpret_cast<unsigned*>(p->d_part);
+  WMX_CHECK(hipMemset(d_nz, 0, sizeof(unsigned) * p->nb));
+  for (int b = 0; b < p->nb; ++b) WMX_CHECK(launch_cell_sb(p->hb[b], d_nz + b, nullptr));
   WMX_CHECK(hipDeviceSynchronize());
+  {
+    std::vector<unsigned> nz(p->nb);
+    WMX_CHECK(hipMemcpy(nz.data(), d_nz, sizeof(unsigned) * p->nb, hipMemcpyDeviceToHost));
+    for (int b = 0; b < p->nb; ++b)
+      p->hb[b].sb_zero = p->hb[b].Sb ? (int)(~nz[b] & ((1u << NSB_PLANES) - 1u)) : 0;
+    WMX_CHECK(hipMemcpy(p->d_blks, p->hb.data(), sizeof(BlockDev) * p->nb, hipMemcpyHostToDevice));
+  }
   if (p->nd.n_sponge > 0) {
     // rows of each block that need the sponge mean (static geometry; made once)
     WMX_CHECK(hipDeviceSynchronize());

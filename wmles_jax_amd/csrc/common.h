@@ -44,6 +44,8 @@ struct BlockDev {
   float* volf;              // FP32-bulk form: float copy of vol
   double* Sb;               // CFL metric planes (NSB_PLANES: the cell means of the face vectors
                             // (d, x)), or nullptr: taken from S
+  int sb_zero;              // bit i: plane i of Sb is +0.0 in every interior cell (an
+                            // axis-aligned direction), so the dt takes 0.0 without loading it
   double* wall[6];          // WALL_PLANES planes of (n_t1, n_t2) per WallModel face
   double* sponge_mean;      // (5, n0, n1)
   int* sponge_row;          // (n0, n1): 1 if some cell of the row has a sponge weight > 0
@@ -122,7 +124,7 @@ hipError_t launch_inflow(const BlockDev* d_blks, int blk, int N1, int N2, int ng
                          hipStream_t s);
 hipError_t launch_to_f32(const double* a, float* b, long long n, hipStream_t s);
 constexpr int NSB_PLANES = 9;
-hipError_t launch_cell_sb(const BlockDev& hb, hipStream_t s);
+hipError_t launch_cell_sb(const BlockDev& hb, unsigned* d_nz, hipStream_t s);
 hipError_t launch_geometry(const double* X, const int* N, double* vol, double* xc, double* const* S,
                            double* const* xf, hipStream_t s);
 // d_part != nullptr: the launch also writes the CFL partial of each workgroup (step fusion)

@@ -93,12 +93,19 @@ __device__ __forceinline__ void cell_metric(const BlockDev& B, int I, int J, int
   metric_norms(m);
 }
 
-// cell_dt's operands from the stored metric (planes B.Sb) and the volume of cell c
+// cell_dt's operands from the stored metric (planes B.Sb) and the volume of cell c.  A plane
+// that holds +0.0 in every interior cell (B.sb_zero, found at workspace binding) is not read:
+// the operand is that same +0.0, so the arithmetic and its bits are unchanged.  On an
+// axis-aligned block this skips six of the nine planes (48 of the 80 bytes per cell).
 __device__ __forceinline__ void cell_dt_load(const BlockDev& B, unsigned c, double* m, double* vol) {
   const size_t cells = (size_t)B.cells;
+  const unsigned zero = (unsigned)B.sb_zero;
   *vol = ldu(B.vol, c);
 #pragma unroll
-  for (int i = 0; i < NSB_PLANES; ++i) m[i] = ldu(B.Sb + i * cells, c);
+  for (int i = 0; i < NSB_PLANES; ++i) {
+    if ((zero >> i) & 1u) m[i] = 0.0;
+    else m[i] = ldu(B.Sb + i * cells, c);
+  }
   metric_norms(m);
 }
 

@@ -297,23 +297,36 @@ __global__ __launch_bounds__(256) void k_sponge_mean(const BlockDev* __restrict_
 // ----------------------------------------------------------------------- launchers
 // CFL metric of every cell of a block (cell_metric: the cell means of the face vectors and
 // their norms; the face arrays hold N + 1 faces along each direction), made once at
-// workspace binding.
-__global__ __launch_bounds__(256) void k_cell_sb(BlockDev B) {
+// workspace binding.  *nz collects (bitwise OR) the planes that are not +0.0 in some
+// interior cell; the dt skips the loads of the others.
+__global__ __launch_bounds__(256) void k_cell_sb(BlockDev B, unsigned* __restrict__ nz) {
   const unsigned cells = (unsigned)B.cells;
   const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  const unsigned N1 = (unsigned)B.N[1], N2 = (unsigned)B.N[2];
-  const int K = (int)(c % N2), J = (int)((c / N2) % N1), I = (int)(c / (N1 * N2));
-  double m[NSB];
-  cell_metric(B, I, J, K, m);
+  unsigned bits = 0;
+  if (c < cells) {
+    const unsigned N1 = (unsigned)B.N[1], N2 = (unsigned)B.N[2];
+    const int K = (int)(c % N2), J = (int)((c / N2) % N1), I = (int)(c / (N1 * N2));
+    double m[NSB];
+    cell_metric(B, I, J, K, m);
 #pragma unroll
-  for (int i = 0; i < NSB_PLANES; ++i) B.Sb[(size_t)i * cells + c] = m[i];
+    for (int i = 0; i < NSB_PLANES; ++i) B.Sb[(size_t)i * cells + c] = m[i];
+    const int ng = B.ng;
+    if (I >= ng && I < ng + B.n[0] && J >= ng && J < ng + B.n[1] && K >= ng && K < ng + B.n[2]) {
+#pragma unroll
+      for (int i = 0; i < NSB_PLANES; ++i)
+        if (__double_as_longlong(m[i]) != 0) bits |= 1u << i;
+    }
+  }
+  // every lane of the wavefront takes part in the OR; the atomic only for bits the word
+  // does not hold yet (all wavefronts on one word would serialise at the L2)
+  for (int o = 32; o > 0; o >>= 1) bits |= (unsigned)__shfl_xor((int)bits, o, 64);
+  if ((threadIdx.x & 63) == 0 && (bits & ~*(volatile unsigned*)nz) != 0u) atomicOr(nz, bits);
 }
 
-hipError_t launch_cell_sb(const BlockDev& hb, hipStream_t s) {
+hipError_t launch_cell_sb(const BlockDev& hb, unsigned* d_nz, hipStream_t s) {
   if (!hb.Sb) return hipSuccess;
   const long long nb = (hb.cells + 255) / 256;
-  hipLaunchKernelGGL(k_cell_sb, dim3((unsigned)nb), dim3(256), 0, s, hb);
+  hipLaunchKernelGGL(k_cell_sb, dim3((unsigned)nb), dim3(256), 0, s, hb, d_nz);
   return hipGetLastError();
 }
 
