@@ -395,13 +395,14 @@ WMX_HD void shell_cell(unsigned s, const int* N, const int* n, int ng, int* I, i
     return;
   }
   s -= 2u * ung * per_j;
-  // k-slabs: side, then interior (I, J) row, then the ng ghost layers of the row (contiguous
-  // along k, so a wavefront touches ng-cell runs instead of one cell per row)
-  const unsigned per = (unsigned)n[0] * (unsigned)n[1] * ung;
-  const int side = s < per ? 0 : 1;
-  const unsigned t = s - (side ? per : 0u);
-  const unsigned row = t / ung;
-  const int m = (int)(t - row * ung);
+  // k-slabs: interior (I, J) row, then its low side's ng layers, then its high side's.  In
+  // memory a row's high ghosts and the next row's low ghosts are adjacent, so a wavefront
+  // touches runs of 2 ng cells (one cell per row per lane, or one side per row, would touch
+  // one or ng).
+  const unsigned row = s / (2u * ung);
+  const unsigned t = s - row * (2u * ung);
+  const int side = t < ung ? 0 : 1;
+  const int m = (int)(t - (side ? ung : 0u));
   const unsigned i = row / (unsigned)n[1];
   *I = ng + (int)i;
   *J = ng + (int)(row - i * (unsigned)n[1]);
